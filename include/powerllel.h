@@ -1,0 +1,155 @@
+/*
+ * PowerLLEL-MI355X: C ABI of the GPU time-step hot path of the channel-flow
+ * solver (FFT pressure Poisson solve + RK2 projection step) for gfx950.
+ *
+ * Plain C types only (no torch, no HIP types in signatures).  Streams are
+ * passed as `void *` holding a hipStream_t (NULL = the null stream).
+ *
+ * Array conventions are the reference's (xiejb6/PowerLLEL_Channel):
+ *  - a haloed field f(0:nx+1, 0:ny+1, 0:nz+1) is one contiguous double array,
+ *    x fastest: Fortran f(i,j,k) is f[i + (nx+2)*(j + (ny+2)*k)]
+ *    (src/mod_variables.f90:14-44, src/src_c/mod_poisson_solver.c:187-188);
+ *  - dzf_global has nz+2 entries, dzf_global[0] and dzf_global[nz+1] are the
+ *    ghost cell heights (src/mod_mesh.f90:35,156-157).
+ *
+ * Error behaviour: the three drop-in entry points are `void`, like the
+ * reference's; on an error they print a message and abort the process (the
+ * reference aborts through MPI_Abort).  Every other entry point returns
+ * PLL_OK (0) or a negative PLL_E* code; pll_last_error() describes the last
+ * failure of the calling thread.
+ */
+#ifndef POWERLLEL_H
+#define POWERLLEL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PLL_OK = 0,
+    PLL_EINVAL = -1,   /* bad argument (sizes, BC strings, NULL pointers)  */
+    PLL_ENOMEM = -2,   /* device or host allocation failed                 */
+    PLL_EHIP = -3,     /* a HIP runtime call or kernel launch failed       */
+    PLL_ESTATE = -4,   /* called out of order (execute before init, ...)   */
+    PLL_EUNSUP = -5    /* a configuration the library does not implement   */
+};
+
+/* Description of the last error on this thread ("" if none). */
+const char *pll_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Drop-in PPE solver: the reference's three C entry points, same
+ * signatures and semantics, one global instance.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces init_poisson_solver, src/src_c/mod_poisson_solver.c:178-309
+ * (bound from Fortran at src/mod_poissonSolver.f90:61-72, called at
+ * src/main.f90:80-81).  BC strings: bctype_x = bctype_y = "PP";
+ * bctype_z in {"NN","DD","ND","DN"}.  neighbor_xyz[2][4], [2][5] (the
+ * z-pencil's bottom/top ranks) must be MPI_PROC_NULL (any negative value):
+ * the whole domain lives on one GPU. */
+void init_poisson_solver(int nx_global, int ny_global, int nz_global,
+                         double dx, double dy, double *dzf_global,
+                         char bctype_x[], char bctype_y[], char bctype_z[],
+                         int neighbor_xyz[3][6]);
+
+/* Replaces execute_poisson_solver, src/src_c/mod_poisson_solver.c:468-605
+ * (called every step at src/main.f90:177).  `var` is the haloed p array
+ * holding the right-hand side in its interior; on return the interior holds
+ * the solution and the halos are untouched.  `var` may be host memory
+ * (copied to and from the GPU) or device memory (solved in place). */
+void execute_poisson_solver(double *var);
+
+/* Replaces free_poisson_solver, src/src_c/mod_poisson_solver.c:607-632. */
+void free_poisson_solver(void);
+
+/* Status-returning forms of the same three calls (no abort). */
+int init_poisson_solver_status(int nx_global, int ny_global, int nz_global,
+                               double dx, double dy, const double *dzf_global,
+                               const char *bctype_x, const char *bctype_y,
+                               const char *bctype_z, const int neighbor_xyz[3][6]);
+int execute_poisson_solver_status(double *var);
+/* Device-pointer form of execute_poisson_solver on `stream`
+ * (asynchronous: the caller synchronises). */
+int execute_poisson_solver_dev(double *d_var, void *stream);
+int free_poisson_solver_status(void);
+
+/* ------------------------------------------------------------------------
+ * Handle API of the same solver (several independent instances).
+ * ---------------------------------------------------------------------- */
+typedef struct pll_ppe pll_ppe;
+
+/* Same meaning as init_poisson_solver's arguments (one rank, walls in z). */
+int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
+                   const double *dzf_global, const char *bctype_x,
+                   const char *bctype_y, const char *bctype_z);
+/* Solve in place on the device haloed array d_var (interior in/out, halos
+ * untouched), asynchronously on `stream`. */
+int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream);
+/* Per-stage device times (ms) of the last pll_ppe_solve issued while
+ * timing was on: x-fwd, y-fwd, z-solve, y-bwd, x-bwd.  Synchronises. */
+int pll_ppe_set_timing(pll_ppe *h, int on);
+int pll_ppe_stage_times(pll_ppe *h, float ms[5]);
+/* Device bytes held by the handle. */
+size_t pll_ppe_device_bytes(const pll_ppe *h);
+int pll_ppe_destroy(pll_ppe *h);
+
+/* ------------------------------------------------------------------------
+ * Step-level entries: the reference's per-step Fortran routines with the
+ * same array conventions, on device arrays, asynchronous on `stream`.
+ * ---------------------------------------------------------------------- */
+typedef struct pll_channel pll_channel;
+
+/* Grid and physics: dzf_global as above; re_inv = nu
+ * (src/mod_parameters.f90:187); u_crf the convecting-frame velocity;
+ * is_forced / vel_force as the &FORCE namelist (src/mod_calcVel.f90:455-509).
+ * vel_force is used as given (the caller applies transform2CRF's shift,
+ * src/mod_calcVel.f90:47, if u_crf != 0). */
+int pll_channel_create(pll_channel **out, int nx, int ny, int nz, double lx,
+                       double ly, double lz, const double *dzf_global,
+                       double re_inv, double dt, double u_crf,
+                       const int is_forced[3], const double vel_force[3]);
+int pll_channel_destroy(pll_channel *c);
+
+/* timeIntVelRK1_kernel, src/mod_calcVel.f90:52-146 (interior). */
+int pll_channel_rk1(pll_channel *c, const double *u, const double *v,
+                    const double *w, double *u1, double *v1, double *w1,
+                    void *stream);
+/* timeIntVelRK2_kernel, src/mod_calcVel.f90:240-334 (interior, u,v,w in place). */
+int pll_channel_rk2(pll_channel *c, double *u, double *v, double *w,
+                    const double *u1, const double *v1, const double *w1,
+                    void *stream);
+/* calcRHS, src/mod_calcRHS.f90:11-37 (writes the interior of p). */
+int pll_channel_calc_rhs(pll_channel *c, const double *u, const double *v,
+                         const double *w, double *p, void *stream);
+/* correctVel, src/mod_calcVel.f90:428-453. */
+int pll_channel_correct_vel(pll_channel *c, const double *p, double *u,
+                            double *v, double *w, void *stream);
+/* forceVel + Mean_h, src/mod_calcVel.f90:455-509, src/mod_utils.f90:105-131. */
+int pll_channel_force_vel(pll_channel *c, double *u, double *v, double *w,
+                          void *stream);
+/* updateBoundVel at one rank, src/mod_updateBound.f90:30-108. */
+int pll_channel_bc_vel(pll_channel *c, double *u, double *v, double *w,
+                       void *stream);
+/* updateBoundP at one rank, src/mod_updateBound.f90:186-239. */
+int pll_channel_bc_p(pll_channel *c, double *p, void *stream);
+/* Mean_h, src/mod_utils.f90:105-131 (synchronises, result on the host). */
+int pll_channel_mean_h(pll_channel *c, const double *q, double *mean,
+                       void *stream);
+/* calcMaxDiv, src/mod_utils.f90:257-318: max |div u| and its 1-based
+ * (i,j,k) (first in k,j,i order among equal maxima).  Synchronises. */
+int pll_channel_max_div(pll_channel *c, const double *u, const double *v,
+                        const double *w, double *value, int ijk[3],
+                        void *stream);
+/* One whole time step, src/main.f90:137-208: RK1, BC, RK2, BC, calcRHS,
+ * PPE (ppe must match the grid), updateBoundP, correctVel, forceVel, BC. */
+int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v,
+                     double *w, double *u1, double *v1, double *w1, double *p,
+                     void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

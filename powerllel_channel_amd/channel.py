@@ -1,0 +1,110 @@
+"""Python mirror of the reference's per-step routines on the gfx950 library.
+
+Method names follow the Fortran subroutines of src/main.f90:131-212:
+rk1 (timeIntVelRK1), rk2 (timeIntVelRK2), calc_rhs (calcRHS), correct_vel
+(correctVel), force_vel (forceVel + Mean_h), bc_vel (updateBoundVel), bc_p
+(updateBoundP), max_div (calcMaxDiv), step (the whole loop body).
+All fields are contiguous float64 CUDA tensors of shape (nz+2, ny+2, nx+2).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+from .ppe import PoissonSolver, _stream_ptr
+
+
+def _p(t):
+    import torch
+    if (not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_contiguous()
+            or t.device.type != "cuda"):
+        raise ValueError("fields must be contiguous float64 CUDA tensors")
+    return t.data_ptr()
+
+
+class Channel:
+    def __init__(self, nx, ny, nz, lx, ly, lz, dzf_global, nu, dt, u_crf=0.0,
+                 is_forced=(True, False, False), vel_force=(1.0, 0.0, 0.0)):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.lx, self.ly, self.lz = float(lx), float(ly), float(lz)
+        self.dzf = np.ascontiguousarray(dzf_global, dtype=np.float64)
+        self.nu, self.dt, self.u_crf = float(nu), float(dt), float(u_crf)
+        h = C.c_void_p()
+        isf = (C.c_int * 3)(*[1 if x else 0 for x in is_forced])
+        vf = (C.c_double * 3)(*[float(x) for x in vel_force])
+        check(lib().pll_channel_create(C.byref(h), self.nx, self.ny, self.nz, self.lx, self.ly,
+                                       self.lz, self.dzf.ctypes.data, self.nu, self.dt,
+                                       self.u_crf, isf, vf))
+        self._h = h
+
+    @property
+    def shape(self):
+        return (self.nz + 2, self.ny + 2, self.nx + 2)
+
+    def _chk(self, *ts):
+        for t in ts:
+            if tuple(t.shape) != self.shape:
+                raise ValueError(f"field shape {tuple(t.shape)} != {self.shape}")
+
+    def rk1(self, u, v, w, u1, v1, w1, stream=None):
+        self._chk(u, v, w, u1, v1, w1)
+        check(lib().pll_channel_rk1(self._h, _p(u), _p(v), _p(w), _p(u1), _p(v1), _p(w1),
+                                    _stream_ptr(stream)))
+
+    def rk2(self, u, v, w, u1, v1, w1, stream=None):
+        self._chk(u, v, w, u1, v1, w1)
+        check(lib().pll_channel_rk2(self._h, _p(u), _p(v), _p(w), _p(u1), _p(v1), _p(w1),
+                                    _stream_ptr(stream)))
+
+    def calc_rhs(self, u, v, w, p, stream=None):
+        self._chk(u, v, w, p)
+        check(lib().pll_channel_calc_rhs(self._h, _p(u), _p(v), _p(w), _p(p), _stream_ptr(stream)))
+
+    def correct_vel(self, p, u, v, w, stream=None):
+        self._chk(u, v, w, p)
+        check(lib().pll_channel_correct_vel(self._h, _p(p), _p(u), _p(v), _p(w),
+                                            _stream_ptr(stream)))
+
+    def force_vel(self, u, v, w, stream=None):
+        self._chk(u, v, w)
+        check(lib().pll_channel_force_vel(self._h, _p(u), _p(v), _p(w), _stream_ptr(stream)))
+
+    def bc_vel(self, u, v, w, stream=None):
+        self._chk(u, v, w)
+        check(lib().pll_channel_bc_vel(self._h, _p(u), _p(v), _p(w), _stream_ptr(stream)))
+
+    def bc_p(self, p, stream=None):
+        self._chk(p)
+        check(lib().pll_channel_bc_p(self._h, _p(p), _stream_ptr(stream)))
+
+    def mean_h(self, q, stream=None) -> float:
+        self._chk(q)
+        out = C.c_double()
+        check(lib().pll_channel_mean_h(self._h, _p(q), C.byref(out), _stream_ptr(stream)))
+        return out.value
+
+    def max_div(self, u, v, w, stream=None):
+        self._chk(u, v, w)
+        val = C.c_double()
+        ijk = (C.c_int * 3)()
+        check(lib().pll_channel_max_div(self._h, _p(u), _p(v), _p(w), C.byref(val), ijk,
+                                        _stream_ptr(stream)))
+        return val.value, (ijk[0], ijk[1], ijk[2])
+
+    def step(self, ppe: PoissonSolver, u, v, w, u1, v1, w1, p, stream=None):
+        self._chk(u, v, w, u1, v1, w1, p)
+        check(lib().pll_channel_step(self._h, ppe.handle, _p(u), _p(v), _p(w), _p(u1), _p(v1),
+                                     _p(w1), _p(p), _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().pll_channel_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,95 @@
+/*
+ * Internal interface between the C host layer (ppe.c, channel.c) and the HIP
+ * kernel launchers (*.hip).  Plain C structs; device pointers as void*.
+ */
+#ifndef PLL_INTERNAL_H
+#define PLL_INTERNAL_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLL_MAX_STAGES 12
+
+/* One complex FFT of length L done by T threads holding E points each
+ * (L = T*E), in nst radix stages rad[0..nst-1] (each radix divides E). */
+typedef struct {
+    int L, E, T, nst;
+    int rad[PLL_MAX_STAGES];
+} pll_fftplan;
+
+/* Everything the PPE kernels need, all pointers on the device.
+ *
+ * Spectral work array W (N doubles = L*ny*nz complex, L = nx/2), layout
+ * W[k][my][mx], mx fastest.  After the x transform, slot mx of a row holds
+ * X_mx for 1 <= mx < L and (X_0, X_L) packed in slot 0.  After the y
+ * transform, columns mx >= 1 hold the complex DFT along y; column 0 holds
+ * A = DFT_y(X_0) and B = DFT_y(X_L) (both Hermitian) packed as
+ *   slot 0: (A_0, B_0), slot ny/2: (A_{ny/2}, B_{ny/2}),
+ *   slot my in (0, ny/2): A_my,  slot ny-my: B_my.
+ * Every real number of W is then one independent tridiagonal system in z
+ * with eigenvalue lambda_x(ix) + lambda_y(iy) (ix in [0,L], iy in [0,ny/2]). */
+typedef struct {
+    int nx, ny, nz, L;
+    int pin_gauge;             /* 1: (0,0) mode top value fixed at 0 */
+    double normfactor;         /* 1/(nx*ny) */
+    pll_fftplan px, py;
+    int x_rows_per_wg;         /* rows per workgroup, x kernels     */
+    int y_cols_per_wg;         /* columns per workgroup, y kernels  */
+    const void *tw_x;          /* double2[nx]: exp(-2 pi i t / nx)  */
+    const void *tw_y;          /* double2[ny]: exp(-2 pi i t / ny)  */
+    const double *a, *c;       /* [nz] tridiagonal off-diagonals    */
+    const double *piv;         /* [nz][ny/2+1][L+1] factorised diagonal */
+} pll_ppe_dev;
+
+int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s);
+int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s);
+int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s);
+int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s);
+int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s);
+/* 1 if a kernel instantiation exists for E (elements per thread). */
+int pll_fft_supported_E(int E);
+/* largest workgroup the E instantiation is compiled for */
+int pll_fft_max_threads(int E);
+
+/* Grid constants for the step kernels (device pointers for z arrays). */
+typedef struct {
+    int nx, ny, nz;
+    double dx_inv, dy_inv, re_inv, dt, u_crf;
+    const double *dzf, *dzf_inv, *dzc_inv, *dzflzi; /* [nz+2] */
+} pll_grid_dev;
+
+int pll_launch_rk1(const pll_grid_dev *g, const double *u, const double *v,
+                   const double *w, double *u1, double *v1, double *w1, hipStream_t s);
+int pll_launch_rk2(const pll_grid_dev *g, double *u, double *v, double *w,
+                   const double *u1, const double *v1, const double *w1, hipStream_t s);
+int pll_launch_calc_rhs(const pll_grid_dev *g, const double *u, const double *v,
+                        const double *w, double *p, hipStream_t s);
+int pll_launch_correct_vel(const pll_grid_dev *g, const double *p, double *u,
+                           double *v, double *w, hipStream_t s);
+/* partial sums of q*dzflzi per (j,k) row into part[ny*nz]; then a single
+ * workgroup reduces them to out[0] = sum / nx / ny (Mean_h). */
+int pll_launch_mean_partials(const pll_grid_dev *g, const double *q, double *part,
+                             hipStream_t s);
+int pll_launch_mean_final(int n, const double *part, double *out, int nx, int ny,
+                          hipStream_t s);
+/* q += (vel_force - mean) on the interior, mean read from device *mean. */
+int pll_launch_force_shift(const pll_grid_dev *g, double *q, double vel_force,
+                           const double *mean, hipStream_t s);
+int pll_launch_bc_vel(const pll_grid_dev *g, double *u, double *v, double *w,
+                      hipStream_t s);
+int pll_launch_bc_p(const pll_grid_dev *g, double *p, hipStream_t s);
+/* per-(j,k)-row max |div| with linear index; then final reduction. */
+int pll_launch_maxdiv_partials(const pll_grid_dev *g, const double *u,
+                               const double *v, const double *w, double *pval,
+                               long long *pidx, hipStream_t s);
+int pll_launch_maxdiv_final(int n, const double *pval, const long long *pidx,
+                            double *oval, long long *oidx, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

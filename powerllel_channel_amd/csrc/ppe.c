@@ -1,0 +1,434 @@
+/*
+ * Host side of the GPU pressure Poisson solver (C, drives the HIP kernels).
+ *
+ * Mirrors src/src_c/mod_poisson_solver.c of the reference:
+ *   init_poisson_solver  :178-309  -> pll_ppe_create + the global instance
+ *   execute_poisson_solver :468-605 -> pll_ppe_solve (5 kernels)
+ *   free_poisson_solver  :607-632  -> pll_ppe_destroy
+ * The eigenvalues follow get_eigen_values (src/src_c/mod_fft_fftw.inc:175-198)
+ * and the z coefficients set_trid_coeff (mod_poisson_solver.c:40-103), both
+ * evaluated here with the reference's formulas.  The solver state stays on
+ * the GPU for the lifetime of the handle.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/powerllel.h"
+#include "pll_internal.h"
+
+/* ------------------------------------------------------------------ errors */
+static __thread char g_err[512];
+
+const char *pll_last_error(void) { return g_err; }
+
+int pll_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return pll_set_error(PLL_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+/* ------------------------------------------------------------------ plans */
+/* Factor L = 2^a 3^b into radix stages from {8,6,4,3,2}, each dividing E,
+ * taking the first E in {12,8,24,6,4,3,2} that divides L (E = 24 needs more
+ * than 256 VGPRs per lane, so it is the last resort of the large ones). */
+static int make_plan(int L, pll_fftplan *p)
+{
+    memset(p, 0, sizeof *p);
+    int a2 = 0, b3 = 0, m = L;
+    if (L < 2) return pll_set_error(PLL_EUNSUP, "FFT length %d < 2", L);
+    while (m % 2 == 0) { m /= 2; a2++; }
+    while (m % 3 == 0) { m /= 3; b3++; }
+    if (m != 1)
+        return pll_set_error(PLL_EUNSUP, "FFT length %d has a prime factor > 3", L);
+    static const int Es[] = {12, 8, 24, 6, 4, 3, 2};
+    for (size_t ie = 0; ie < sizeof Es / sizeof Es[0]; ie++) {
+        int E = Es[ie];
+        if (L % E != 0 || !pll_fft_supported_E(E)) continue;
+        int a = a2, b = b3, n = 0, prod = 1, ok = 1;
+        int rad[PLL_MAX_STAGES];
+        while (a >= 3 && E % 8 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 8; a -= 3; }
+        while (a >= 2 && E % 4 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 4; a -= 2; }
+        if (a == 1 && b >= 1 && E % 6 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 6; a = 0; b--; }
+        while (a >= 1 && E % 2 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 2; a -= 1; }
+        while (b >= 1 && E % 3 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 3; b -= 1; }
+        if (a != 0 || b != 0) ok = 0;
+        for (int s = 0; s < n; s++) prod *= rad[s];
+        if (!ok || prod != L) continue;
+        p->L = L;
+        p->E = E;
+        p->T = L / E;
+        p->nst = n;
+        memcpy(p->rad, rad, sizeof(int) * (size_t)n);
+        return PLL_OK;
+    }
+    return pll_set_error(PLL_EUNSUP, "no FFT plan for length %d", L);
+}
+
+/* exp(-2 pi i t / n), t in [0, n), as interleaved (re, im) doubles */
+static double *twiddles(int n)
+{
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    if (!t) return NULL;
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int i = 0; i < n; i++) {
+        long double ang = twopi * (long double)i / (long double)n;
+        t[2 * i] = (double)cosl(ang);
+        t[2 * i + 1] = (double)(-sinl(ang));
+    }
+    return t;
+}
+
+/* get_eigen_values, mod_fft_fftw.inc:175-183, periodic case, 1-based i */
+static void eigen_pp(int n, double h, double *lam)
+{
+    const double pi = acos(-1.0);
+    for (int i = 1; i <= n; i++) lam[i - 1] = 2.0 * (cos(2.0 * pi * (i - 1.0) / n) - 1.0);
+    for (int i = 0; i < n; i++) lam[i] /= (h * h); /* mod_poisson_solver.c:231-237 */
+}
+
+/* set_trid_coeff, mod_poisson_solver.c:40-103, 'c' grid, walls at both ends.
+ * dzf: n+2 entries with ghosts. */
+static int trid_coeff(int n, const double *dzf, const char *bct, double *a, double *b,
+                      double *c)
+{
+    for (int k = 0; k < n; k++) {
+        a[k] = 2.0 / (dzf[k + 1] * (dzf[k] + dzf[k + 1]));
+        c[k] = 2.0 / (dzf[k + 1] * (dzf[k + 2] + dzf[k + 1]));
+    }
+    for (int k = 0; k < n; k++) b[k] = -a[k] - c[k];
+    for (int side = 0; side < 2; side++) {
+        char bc = bct[side];
+        double factor;
+        if (bc == 'D') factor = -1.0;
+        else if (bc == 'N') factor = 1.0;
+        else return pll_set_error(PLL_EUNSUP, "z boundary type '%c' (want N or D)", bc);
+        if (side == 0) {
+            b[0] += factor * a[0];
+            a[0] *= fabs(factor) - 1.0;
+        } else {
+            b[n - 1] += factor * c[n - 1];
+            c[n - 1] *= fabs(factor) - 1.0;
+        }
+    }
+    return PLL_OK;
+}
+
+/* ------------------------------------------------------------------ handle */
+struct pll_ppe {
+    int nx, ny, nz, L;
+    pll_ppe_dev d;
+    void *W, *tw_x, *tw_y, *a, *c, *piv;
+    double *stage;   /* haloed staging buffer for host-pointer calls */
+    size_t stage_bytes;
+    size_t dev_bytes;
+    int timing;
+    hipEvent_t ev[6];
+};
+
+static int dmalloc(pll_ppe *h, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = NULL;
+        return pll_set_error(PLL_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    h->dev_bytes += bytes;
+    return PLL_OK;
+}
+
+int pll_ppe_destroy(pll_ppe *h)
+{
+    if (!h) return PLL_OK;
+    void *bufs[] = {h->W, h->tw_x, h->tw_y, h->a, h->c, h->piv, h->stage};
+    for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
+        if (bufs[i]) hipFree(bufs[i]);
+    if (h->timing)
+        for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[i]);
+    free(h);
+    return PLL_OK;
+}
+
+static int choose_launch(pll_ppe *h)
+{
+    /* x kernels: rows per workgroup, <= max threads and <= 64 KiB LDS */
+    const pll_fftplan *px = &h->d.px, *py = &h->d.py;
+    int maxt = pll_fft_max_threads(px->E);
+    int rw = 256 / px->T;
+    if (rw < 1) rw = 1;
+    while (rw > 1 && (rw * px->T > maxt || (size_t)rw * (px->L + 1) * 16 > 65536)) rw--;
+    if (rw * px->T > maxt || (size_t)rw * (px->L + 1) * 16 > 65536)
+        return pll_set_error(PLL_EUNSUP, "nx=%d too large for the x transform kernel", h->nx);
+    h->d.x_rows_per_wg = rw;
+    /* y kernels: adjacent columns per workgroup (>= 8 gives 128-byte rows) */
+    maxt = pll_fft_max_threads(py->E);
+    int cols = 8;
+    if (cols > h->L) cols = h->L;
+    while (cols > 1 && (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 65536)) cols--;
+    if (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 65536)
+        return pll_set_error(PLL_EUNSUP, "ny=%d too large for the y transform kernel", h->ny);
+    h->d.y_cols_per_wg = cols;
+    return PLL_OK;
+}
+
+int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
+                   const double *dzf_global, const char *bctype_x, const char *bctype_y,
+                   const char *bctype_z)
+{
+    if (!out) return pll_set_error(PLL_EINVAL, "out is NULL");
+    *out = NULL;
+    if (!dzf_global || !bctype_x || !bctype_y || !bctype_z)
+        return pll_set_error(PLL_EINVAL, "NULL argument");
+    if (strcmp(bctype_x, "PP") != 0 || strcmp(bctype_y, "PP") != 0)
+        return pll_set_error(PLL_EUNSUP, "x and y must be periodic (\"PP\"), got \"%s\",\"%s\"",
+                             bctype_x, bctype_y);
+    if (strlen(bctype_z) != 2)
+        return pll_set_error(PLL_EINVAL, "bctype_z must have 2 characters");
+    if (nx < 4 || ny < 2 || nz < 2 || nx % 2 || ny % 2)
+        return pll_set_error(PLL_EINVAL, "need even nx >= 4, even ny >= 2, nz >= 2 (got %d %d %d)",
+                             nx, ny, nz);
+    if (!(dx > 0.0) || !(dy > 0.0)) return pll_set_error(PLL_EINVAL, "dx, dy must be > 0");
+    for (int k = 0; k < nz + 2; k++)
+        if (!(dzf_global[k] > 0.0)) return pll_set_error(PLL_EINVAL, "dzf_global[%d] <= 0", k);
+
+    pll_ppe *h = (pll_ppe *)calloc(1, sizeof *h);
+    if (!h) return pll_set_error(PLL_ENOMEM, "calloc failed");
+    h->nx = nx;
+    h->ny = ny;
+    h->nz = nz;
+    h->L = nx / 2;
+    int rc;
+    if ((rc = make_plan(h->L, &h->d.px)) || (rc = make_plan(ny, &h->d.py)) ||
+        (rc = choose_launch(h))) {
+        free(h);
+        return rc;
+    }
+    h->d.nx = nx;
+    h->d.ny = ny;
+    h->d.nz = nz;
+    h->d.L = h->L;
+    h->d.normfactor = 1.0 / (1.0 * nx * 1.0 * ny); /* mod_fft_fftw.inc:154 */
+    h->d.pin_gauge = (bctype_z[0] == 'N' && bctype_z[1] == 'N');
+
+    const int L = h->L, nyh = ny / 2 + 1;
+    const size_t N = (size_t)nx * ny * nz;
+    const size_t npiv = (size_t)nz * nyh * (L + 1);
+    double *a = (double *)malloc(sizeof(double) * (size_t)nz);
+    double *c = (double *)malloc(sizeof(double) * (size_t)nz);
+    double *bz = (double *)malloc(sizeof(double) * (size_t)nz);
+    double *lx = (double *)malloc(sizeof(double) * (size_t)nx);
+    double *ly = (double *)malloc(sizeof(double) * (size_t)ny);
+    double *piv = (double *)malloc(sizeof(double) * npiv);
+    double *twx = twiddles(nx), *twy = twiddles(ny);
+    if (!a || !c || !bz || !lx || !ly || !piv || !twx || !twy) {
+        rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
+        goto fail;
+    }
+    if ((rc = trid_coeff(nz, dzf_global, bctype_z, a, bz, c))) goto fail;
+    eigen_pp(nx, dx, lx);
+    eigen_pp(ny, dy, ly);
+    /* pivots: b_k = bz_k + lambda; b_k -= (a_k / b_{k-1}) c_{k-1}
+     * (mod_poisson_solver.c:257-275) for every distinct lambda */
+    for (int iy = 0; iy < nyh; iy++) {
+        for (int ix = 0; ix <= L; ix++) {
+            const double lam = lx[ix] + ly[iy];
+            const size_t col = (size_t)iy * (L + 1) + ix;
+            double bprev = bz[0] + lam;
+            piv[col] = bprev;
+            for (int k = 1; k < nz; k++) {
+                double bk = bz[k] + lam;
+                double a_tmp = a[k] / bprev;
+                bk -= (a_tmp * c[k - 1]);
+                piv[(size_t)k * nyh * (L + 1) + col] = bk;
+                bprev = bk;
+            }
+        }
+    }
+    if ((rc = dmalloc(h, &h->W, sizeof(double) * N)) ||
+        (rc = dmalloc(h, &h->tw_x, sizeof(double) * 2 * (size_t)nx)) ||
+        (rc = dmalloc(h, &h->tw_y, sizeof(double) * 2 * (size_t)ny)) ||
+        (rc = dmalloc(h, &h->a, sizeof(double) * (size_t)nz)) ||
+        (rc = dmalloc(h, &h->c, sizeof(double) * (size_t)nz)) ||
+        (rc = dmalloc(h, &h->piv, sizeof(double) * npiv)))
+        goto fail;
+#define H2D(dst, src, bytes)                                                              \
+    do {                                                                                  \
+        hipError_t e_ = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);                \
+        if (e_ != hipSuccess) {                                                           \
+            rc = pll_set_error(PLL_EHIP, "hipMemcpy H2D failed: %s", hipGetErrorString(e_)); \
+            goto fail;                                                                    \
+        }                                                                                 \
+    } while (0)
+    H2D(h->tw_x, twx, sizeof(double) * 2 * (size_t)nx);
+    H2D(h->tw_y, twy, sizeof(double) * 2 * (size_t)ny);
+    H2D(h->a, a, sizeof(double) * (size_t)nz);
+    H2D(h->c, c, sizeof(double) * (size_t)nz);
+    H2D(h->piv, piv, sizeof(double) * npiv);
+#undef H2D
+    h->d.tw_x = h->tw_x;
+    h->d.tw_y = h->tw_y;
+    h->d.a = (const double *)h->a;
+    h->d.c = (const double *)h->c;
+    h->d.piv = (const double *)h->piv;
+    free(a); free(c); free(bz); free(lx); free(ly); free(piv); free(twx); free(twy);
+    *out = h;
+    return PLL_OK;
+fail:
+    free(a); free(c); free(bz); free(lx); free(ly); free(piv); free(twx); free(twy);
+    pll_ppe_destroy(h);
+    return rc;
+}
+
+size_t pll_ppe_device_bytes(const pll_ppe *h) { return h ? h->dev_bytes : 0; }
+
+int pll_ppe_set_timing(pll_ppe *h, int on)
+{
+    if (!h) return pll_set_error(PLL_EINVAL, "NULL handle");
+    if (on && !h->timing) {
+        for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&h->ev[i]));
+        h->timing = 1;
+    } else if (!on && h->timing) {
+        for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[i]);
+        h->timing = 0;
+    }
+    return PLL_OK;
+}
+
+int pll_ppe_stage_times(pll_ppe *h, float ms[5])
+{
+    if (!h || !ms) return pll_set_error(PLL_EINVAL, "NULL argument");
+    if (!h->timing) return pll_set_error(PLL_ESTATE, "timing is off");
+    HIPCHK(hipEventSynchronize(h->ev[5]));
+    for (int i = 0; i < 5; i++) HIPCHK(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+    return PLL_OK;
+}
+
+static int launch_rc(int rc, const char *what)
+{
+    if (rc == 0) return PLL_OK;
+    if (rc == -1) return pll_set_error(PLL_EUNSUP, "%s: launch shape out of range", what);
+    if (rc == -5) return pll_set_error(PLL_EUNSUP, "%s: no kernel for this FFT size", what);
+    return pll_set_error(PLL_EHIP, "%s: kernel launch failed: %s", what,
+                         hipGetErrorString(hipGetLastError()));
+}
+
+int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
+{
+    if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
+    if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[0], s));
+    if ((rc = launch_rc(pll_launch_xfwd(&h->d, d_var, h->W, s), "x forward"))) return rc;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[1], s));
+    if ((rc = launch_rc(pll_launch_yfwd(&h->d, h->W, s), "y forward"))) return rc;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[2], s));
+    if ((rc = launch_rc(pll_launch_trid(&h->d, h->W, s), "z solve"))) return rc;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[3], s));
+    if ((rc = launch_rc(pll_launch_ybwd(&h->d, h->W, s), "y backward"))) return rc;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[4], s));
+    if ((rc = launch_rc(pll_launch_xbwd(&h->d, h->W, d_var, s), "x backward"))) return rc;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[5], s));
+    return PLL_OK;
+}
+
+/* ------------------------------------------------------------------ drop-in */
+static pll_ppe *g_ppe = NULL;
+
+int init_poisson_solver_status(int nx_global, int ny_global, int nz_global, double dx,
+                               double dy, const double *dzf_global, const char *bctype_x,
+                               const char *bctype_y, const char *bctype_z,
+                               const int neighbor_xyz[3][6])
+{
+    if (g_ppe) return pll_set_error(PLL_ESTATE, "init_poisson_solver called twice without free");
+    if (neighbor_xyz && (neighbor_xyz[2][4] >= 0 || neighbor_xyz[2][5] >= 0))
+        return pll_set_error(PLL_EUNSUP,
+                             "z-pencil neighbours (%d, %d) are ranks: the GPU solver holds the "
+                             "whole domain and needs walls (MPI_PROC_NULL) at both z ends",
+                             neighbor_xyz[2][4], neighbor_xyz[2][5]);
+    return pll_ppe_create(&g_ppe, nx_global, ny_global, nz_global, dx, dy, dzf_global, bctype_x,
+                          bctype_y, bctype_z);
+}
+
+static int is_device_ptr(const void *p)
+{
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+int execute_poisson_solver_dev(double *d_var, void *stream)
+{
+    if (!g_ppe) return pll_set_error(PLL_ESTATE, "execute_poisson_solver before init");
+    return pll_ppe_solve(g_ppe, d_var, stream);
+}
+
+int execute_poisson_solver_status(double *var)
+{
+    pll_ppe *h = g_ppe;
+    if (!h) return pll_set_error(PLL_ESTATE, "execute_poisson_solver before init");
+    if (!var) return pll_set_error(PLL_EINVAL, "var is NULL");
+    int rc;
+    if (is_device_ptr(var)) {
+        if ((rc = pll_ppe_solve(h, var, NULL))) return rc;
+        HIPCHK(hipStreamSynchronize(NULL));
+        return PLL_OK;
+    }
+    const size_t bytes = sizeof(double) * (size_t)(h->nx + 2) * (h->ny + 2) * (h->nz + 2);
+    if (!h->stage) {
+        if ((rc = dmalloc(h, (void **)&h->stage, bytes))) return rc;
+        h->stage_bytes = bytes;
+    }
+    HIPCHK(hipMemcpy(h->stage, var, bytes, hipMemcpyHostToDevice));
+    if ((rc = pll_ppe_solve(h, h->stage, NULL))) return rc;
+    HIPCHK(hipMemcpy(var, h->stage, bytes, hipMemcpyDeviceToHost));
+    return PLL_OK;
+}
+
+int free_poisson_solver_status(void)
+{
+    int rc = pll_ppe_destroy(g_ppe);
+    g_ppe = NULL;
+    return rc;
+}
+
+static void abort_on(int rc, const char *what)
+{
+    if (rc == PLL_OK) return;
+    fprintf(stderr, "PowerLLEL-MI355X.ERROR: %s: %s (code %d)\n", what, pll_last_error(), rc);
+    fflush(stderr);
+    abort();
+}
+
+void init_poisson_solver(int nx_global, int ny_global, int nz_global, double dx, double dy,
+                         double *dzf_global, char bctype_x[], char bctype_y[], char bctype_z[],
+                         int neighbor_xyz[3][6])
+{
+    abort_on(init_poisson_solver_status(nx_global, ny_global, nz_global, dx, dy, dzf_global,
+                                        bctype_x, bctype_y, bctype_z,
+                                        (const int(*)[6])neighbor_xyz),
+             "init_poisson_solver");
+}
+
+void execute_poisson_solver(double *var)
+{
+    abort_on(execute_poisson_solver_status(var), "execute_poisson_solver");
+}
+
+void free_poisson_solver(void) { abort_on(free_poisson_solver_status(), "free_poisson_solver"); }
