@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the channel-flow hot path on one MI355X.
+
+Headline: FP64 pressure Poisson solves per second at 768x384x256 (Lx = 4 pi,
+Ly = 2 pi, Lz = 2, tanh gamma = 2.0, seeded spectral right-hand side), i.e.
+the reference's execute_poisson_solver on that grid.  One "step" of the
+timed loop is one in-place solve (the solution of one solve is the
+right-hand side of the next; inputs stay resident in HBM).  Also reported:
+RK2 projection steps per second at the same grid (nu = 1/5600, dt = 1e-4,
+seeded perturbed Poiseuille start), the HBM roofline of the dominant PPE
+kernel from HIP events on the solve's stream, and the oracle (the C/OpenMP
+restatement of the reference algorithm) timed on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--ppe-only]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD spec
+METRIC = "PPE solves/s + RK2 steps/s @768x384x256, FP32-refined vs FP64, % HBM peak"
+NX, NY, NZ, GAMMA = 768, 384, 256, 2.0
+STAGES = ["x_fwd", "y_fwd", "z_solve", "y_bwd", "x_bwd"]
+# algorithmic bytes per cell of each PPE stage: read + write N doubles once
+STAGE_BYTES_PER_CELL = {s: 16 for s in STAGES}
+PPE_BYTES_PER_CELL = 80
+# RK1 48 + RK2 72 + calcRHS 32 + PPE 80 + correctVel 56 + forceVel 24
+STEP_BYTES_PER_CELL = 312
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_ppe_768x384x256.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch per kernel, measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration)."""
+    out = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return out
+    for line in open(TRAFFIC_FILE):
+        line = line.strip()
+        if not line or line.startswith("#") or "=" not in line:
+            continue
+        k, v = line.split("=", 1)
+        try:
+            out[k.strip()] = float(v)
+        except ValueError:
+            pass
+    return out
+
+
+def bench_ppe(torch, dev, steps, warmup):
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, spectral_rhs
+    g = Grid(NX, NY, NZ, LX, LY, LZ, 2.0 * GAMMA)
+    log(f"[bench] generating spectral rhs {NX}x{NY}x{NZ}")
+    rhs = spectral_rhs(g, 0)
+    p = torch.from_numpy(rhs).to(dev)
+    del rhs
+    s = PoissonSolver(NX, NY, NZ, g.dx, g.dy, g.dzf)
+    for _ in range(warmup):
+        s.solve(p)
+    torch.cuda.synchronize()
+    s.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.solve(p)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stage_ms, nrec = s.stage_times_mean()
+    s.set_timing(False)
+    assert bool(torch.isfinite(p).all()), "non-finite values after the timed solves"
+    ms = (t1 - t0) * 1e3 / steps
+    log(f"[bench] PPE {ms:.3f} ms/solve; stages " +
+        ", ".join(f"{n} {t:.3f}" for n, t in zip(STAGES, stage_ms)))
+    dev_bytes = s.device_bytes()
+    s.close()
+    del p
+    return ms, dict(zip(STAGES, stage_ms)), nrec, dev_bytes
+
+
+def bench_step(torch, dev, steps, warmup):
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields
+    nu, dt = 1.0 / 5600, 1e-4
+    g = Grid(NX, NY, NZ, LX, LY, LZ, 2.0 * GAMMA)
+    log("[bench] generating perturbed Poiseuille start")
+    u, v, w = poiseuille_fields(g, 0)
+    t = [torch.from_numpy(a).to(dev) for a in (u, v, w)]
+    del u, v, w
+    t += [torch.zeros(g.shape, dtype=torch.float64, device=dev) for _ in range(4)]
+    ch = Channel(NX, NY, NZ, LX, LY, LZ, g.dzf, nu, dt)
+    s = PoissonSolver(NX, NY, NZ, g.dx, g.dy, g.dzf)
+    ch.bc_vel(t[0], t[1], t[2])
+    for _ in range(warmup):
+        ch.step(s, *t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ch.step(s, *t)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    div, _ = ch.max_div(t[0], t[1], t[2])
+    ms = (t1 - t0) * 1e3 / steps
+    log(f"[bench] RK2 step {ms:.3f} ms/step; max|div u| = {div:.2e}")
+    ch.close()
+    s.close()
+    del t
+    return ms, div
+
+
+def copy_bandwidth(torch, dev):
+    """Device-to-device copy of 2 GiB (read + write), for context."""
+    n = 2 * 1024 ** 3 // 8
+    a = torch.empty(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    del a, b
+    return 2 * n * 8 / dt / 1e9
+
+
+def cpu_baseline(ppe_solves=5, steps=2):
+    """The oracle (C/OpenMP restatement of the reference CPU algorithm, in-repo
+    FFT in place of FFTW) on the host cores: PPE solves and RK2 steps at
+    768x384x256 after one warm-up each."""
+    from oracle import oracle as O
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields, spectral_rhs
+    O.build()
+    threads = O.get_threads()
+    g = Grid(NX, NY, NZ, LX, LY, LZ, 2.0 * GAMMA)
+    p = spectral_rhs(g, 0)
+    ppe = O.OraclePPE(NX, NY, NZ, g.dx, g.dy, g.dzf, pin_gauge=True)
+    ppe.solve(p)
+    t0 = time.perf_counter()
+    for _ in range(ppe_solves):
+        ppe.solve(p)
+    t_ppe = (time.perf_counter() - t0) / ppe_solves
+    log(f"[bench] oracle PPE {t_ppe:.3f} s/solve on {threads} threads")
+    G = O.OracleGrid(NX, NY, NZ, LX, LY, LZ, g.dzf, 1.0 / 5600, 1e-4)
+    u, v, w = poiseuille_fields(g, 0)
+    u1, v1, w1 = (np.zeros(g.shape) for _ in range(3))
+    O.bc_vel(G, u, v, w, 0.0)
+    O.step(G, ppe, u, v, w, u1, v1, w1, p)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.step(G, ppe, u, v, w, u1, v1, w1, p)
+    t_step = (time.perf_counter() - t0) / steps
+    log(f"[bench] oracle RK2 step {t_step:.3f} s/step")
+    ppe.close()
+    return {
+        "value": 1.0 / t_ppe,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{ppe_solves} FP64 PPE solves and {steps} RK2 step(s) at {NX}x{NY}x{NZ} "
+                   "after one warm-up each; oracle = C/OpenMP restatement of the reference "
+                   "CPU algorithm with an in-repo FFT in place of FFTW"),
+        "rk2_steps_per_s": 1.0 / t_step,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--ppe-only", action="store_true", help="skip the RK2 step timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the whole "
+              "domain on one GPU (see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    N = NX * NY * NZ
+
+    ms, stage_ms, nrec, dev_bytes = bench_ppe(torch, dev, args.steps, args.warmup)
+    dom = max(stage_ms, key=stage_ms.get)
+    achieved = STAGE_BYTES_PER_CELL[dom] * N / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = read_traffic().get(dom)
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic, "algorithmic_bytes": STAGE_BYTES_PER_CELL[dom] * N,
+        "mean_ms": round(stage_ms[dom], 4), "timed_launches": nrec,
+    }
+    ppe_gbs = PPE_BYTES_PER_CELL * N / (ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(1e3 / ms, 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded spectral PPE right-hand side; seeded perturbed Poiseuille start",
+        "config": {"workload": "ppe_768x384x256 (FP64 PPE solve; RK2 step at the same grid)",
+                   "grid": [NX, NY, NZ], "lx_ly_lz": ["4pi", "2pi", 2.0], "gamma": GAMMA,
+                   "precision": "fp64"},
+        "roofline": roofline,
+        "ppe_roofline": {"achieved": round(ppe_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ppe_gbs / HBM_PEAK_GBS, 4),
+                         "bytes_per_cell": PPE_BYTES_PER_CELL},
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "ppe_device_bytes": dev_bytes,
+    }
+    if not args.ppe_only:
+        sms, div = bench_step(torch, dev, args.steps, args.warmup)
+        step_gbs = STEP_BYTES_PER_CELL * N / (sms * 1e-3) / 1e9
+        out["rk2_steps_per_s"] = round(1e3 / sms, 2)
+        out["ms_per_rk2_step"] = round(sms, 4)
+        out["rk2_max_div"] = div
+        out["rk2_roofline"] = {"achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                               "bytes_per_cell": STEP_BYTES_PER_CELL}
+    out["copy_bw_GBs"] = round(copy_bandwidth(torch, dev), 1)
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

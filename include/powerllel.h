@@ -88,10 +88,15 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
 /* Solve in place on the device haloed array d_var (interior in/out, halos
  * untouched), asynchronously on `stream`. */
 int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream);
-/* Per-stage device times (ms) of the last pll_ppe_solve issued while
- * timing was on: x-fwd, y-fwd, z-solve, y-bwd, x-bwd.  Synchronises. */
+/* Stage timing with HIP events recorded on the solve's stream (no
+ * counterpart in the reference, which times stages with GPTL,
+ * mod_poisson_solver.c:480-602).  Turning timing on (re)starts the record;
+ * stages are x-fwd, y-fwd, z-solve, y-bwd, x-bwd.  pll_ppe_stage_times
+ * gives the last timed solve, pll_ppe_stage_times_mean the mean over the
+ * last (up to 128) timed solves and their count.  Both synchronise. */
 int pll_ppe_set_timing(pll_ppe *h, int on);
 int pll_ppe_stage_times(pll_ppe *h, float ms[5]);
+int pll_ppe_stage_times_mean(pll_ppe *h, double ms[5], int *nsolves);
 /* Device bytes held by the handle. */
 size_t pll_ppe_device_bytes(const pll_ppe *h);
 int pll_ppe_destroy(pll_ppe *h);

@@ -39,6 +39,11 @@ __device__ __forceinline__ cd rot(cd a)
     return DIR < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
+// Two doubles at an 8-byte (not 16-byte) aligned address: the interior of a
+// haloed row starts one double past the row start, so a pair (x_2m, x_2m+1)
+// is moved as one 16-byte access that is only 8-byte aligned.
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+
 constexpr double kS3 = 0.86602540378443864676372317075294;  // sqrt(3)/2
 constexpr double kH = 0.70710678118654752440084436210485;   // sqrt(2)/2
 
@@ -198,6 +203,84 @@ __device__ __forceinline__ void fft_run(cd (&v)[E], int t, const pll_fftplan &P,
     }
 }
 
+// Compile-time form of the same stages: radices, Ns and T are constants, so
+// the butterfly indices (b mod Ns, output positions, twiddle offsets) fold
+// into shifts and small constants and nothing is spilled between radices.
+template <int R, int E, int DIR, int T, int Ns, int TWSTEP, bool LAST>
+__device__ __forceinline__ void stage_ct(cd (&v)[E], int t, const cd *__restrict__ tw, cd *lds)
+{
+    constexpr int Q = E / R;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = t + q * T;
+        const int k = b % Ns;
+        cd x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cd w = tw[r * k * TWSTEP];
+                if (DIR > 0) w.y = -w.y;
+                x[r] = cmul(x[r], w);
+            }
+        }
+        Dft<R, DIR>::run(x);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
+        } else {
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[base + r * Ns] = x[r];
+        }
+    }
+}
+
+template <int L, int E, int DIR, int TWSCALE, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stages_ct(cd (&v)[E], int t, const cd *__restrict__ tw, cd *lds)
+{
+    static_assert(E % R == 0, "radix must divide E");
+    constexpr int T = L / E;
+    constexpr bool LAST = sizeof...(Rest) == 0;
+    constexpr int TWSTEP = TWSCALE * (L / (Ns * R));
+    stage_ct<R, E, DIR, T, Ns, TWSTEP, LAST>(v, t, tw, lds);
+    if constexpr (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = lds[t + e * T];
+        __syncthreads();
+        stages_ct<L, E, DIR, TWSCALE, Ns * R, Rest...>(v, t, tw, lds);
+    }
+}
+
+// FFT policies used by the kernels: a compile-time plan, or the runtime plan.
+template <int L_, int E_, int... Rs>
+struct CtFFT {
+    static constexpr int E = E_;
+    static __device__ __forceinline__ int threads(const pll_fftplan &) { return L_ / E_; }
+    static __device__ __forceinline__ int length(const pll_fftplan &) { return L_; }
+    template <int DIR, int TWSCALE>
+    static __device__ __forceinline__ void run(cd (&v)[E_], int t, const pll_fftplan &,
+                                               const cd *__restrict__ tw, cd *lds)
+    {
+        stages_ct<L_, E_, DIR, TWSCALE, 1, Rs...>(v, t, tw, lds);
+    }
+};
+
+template <int E_>
+struct RtFFT {
+    static constexpr int E = E_;
+    static __device__ __forceinline__ int threads(const pll_fftplan &P) { return P.T; }
+    static __device__ __forceinline__ int length(const pll_fftplan &P) { return P.L; }
+    template <int DIR, int TWSCALE>
+    static __device__ __forceinline__ void run(cd (&v)[E_], int t, const pll_fftplan &P,
+                                               const cd *__restrict__ tw, cd *lds)
+    {
+        fft_run<E_, DIR>(v, t, P, tw, TWSCALE, lds);
+    }
+};
+
 struct XArgs {
     pll_fftplan P;
     const cd *tw;       // [nx] exp(-2 pi i t/nx)
@@ -218,14 +301,20 @@ constexpr int max_threads()
 {
     return E >= 12 ? 256 : 512;
 }
+template <class F>
+constexpr int max_threads_f()
+{
+    return max_threads<F::E>();
+}
 
 // x forward: haloed real rows -> packed half spectrum rows of W.
-template <int E>
-__global__ void __launch_bounds__(max_threads<E>())
+template <class F>
+__global__ void __launch_bounds__(max_threads_f<F>())
 k_xfwd(const double *__restrict__ p, cd *__restrict__ W, XArgs a)
 {
     extern __shared__ cd lds[];
-    const int T = a.P.T, L = a.P.L;
+    constexpr int E = F::E;
+    const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
     const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
@@ -236,9 +325,14 @@ k_xfwd(const double *__restrict__ p, cd *__restrict__ W, XArgs a)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
-        v[e] = valid ? make_double2(src[2 * m], src[2 * m + 1]) : make_double2(0.0, 0.0);
+        if (valid) {
+            const d2u pr = *reinterpret_cast<const d2u *>(src + 2 * m);
+            v[e] = make_double2(pr.x, pr.y);
+        } else {
+            v[e] = make_double2(0.0, 0.0);
+        }
     }
-    fft_run<E, -1>(v, t, a.P, a.tw, 2, ml);
+    F::template run<-1, 2>(v, t, a.P, a.tw, ml);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
@@ -263,12 +357,13 @@ k_xfwd(const double *__restrict__ p, cd *__restrict__ W, XArgs a)
 }
 
 // x backward: packed half spectrum rows -> haloed real rows * norm.
-template <int E>
-__global__ void __launch_bounds__(max_threads<E>())
+template <class F>
+__global__ void __launch_bounds__(max_threads_f<F>())
 k_xbwd(const cd *__restrict__ W, double *__restrict__ p, XArgs a)
 {
     extern __shared__ cd lds[];
-    const int T = a.P.T, L = a.P.L;
+    constexpr int E = F::E;
+    const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
     const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
@@ -298,25 +393,28 @@ k_xbwd(const cd *__restrict__ W, double *__restrict__ p, XArgs a)
         }
     }
     __syncthreads();
-    fft_run<E, +1>(v, t, a.P, a.tw, 2, ml);
+    F::template run<+1, 2>(v, t, a.P, a.tw, ml);
     if (valid) {
         double *dst = p + (k + 1) * a.off1 + (j + 1) * a.off2 + 1;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int m = t + e * T;
-            dst[2 * m] = v[e].x * a.norm;
-            dst[2 * m + 1] = v[e].y * a.norm;
+            d2u pr;
+            pr.x = v[e].x * a.norm;
+            pr.y = v[e].y * a.norm;
+            *reinterpret_cast<d2u *>(dst + 2 * m) = pr;
         }
     }
 }
 
 // y forward, in place on W: tile of `cols` adjacent columns of one k-plane.
-template <int E>
-__global__ void __launch_bounds__(max_threads<E>())
+template <class F>
+__global__ void __launch_bounds__(max_threads_f<F>())
 k_yfwd(cd *__restrict__ W, YArgs a)
 {
     extern __shared__ cd lds[];
-    const int T = a.P.T, ny = a.ny, L = a.L;
+    constexpr int E = F::E;
+    const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
     const int tile = blockIdx.x % a.ntiles;
     const long long k = blockIdx.x / a.ntiles;
     const int c = threadIdx.x % a.cols, t = threadIdx.x / a.cols;
@@ -330,7 +428,7 @@ k_yfwd(cd *__restrict__ W, YArgs a)
         const int my = t + e * T;
         v[e] = valid ? col[(long long)my * L] : make_double2(0.0, 0.0);
     }
-    fft_run<E, -1>(v, t, a.P, a.tw, 1, ml);
+    F::template run<-1, 1>(v, t, a.P, a.tw, ml);
     if (tile == 0) {
         // column 0 holds DFT_y(X_0 + i X_L): split into A = DFT_y(X_0) and
         // B = DFT_y(X_L), keeping one half of each Hermitian sequence.
@@ -358,12 +456,13 @@ k_yfwd(cd *__restrict__ W, YArgs a)
     }
 }
 
-template <int E>
-__global__ void __launch_bounds__(max_threads<E>())
+template <class F>
+__global__ void __launch_bounds__(max_threads_f<F>())
 k_ybwd(cd *__restrict__ W, YArgs a)
 {
     extern __shared__ cd lds[];
-    const int T = a.P.T, ny = a.ny, L = a.L;
+    constexpr int E = F::E;
+    const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
     const int tile = blockIdx.x % a.ntiles;
     const long long k = blockIdx.x / a.ntiles;
     const int c = threadIdx.x % a.cols, t = threadIdx.x / a.cols;
@@ -399,14 +498,14 @@ k_ybwd(cd *__restrict__ W, YArgs a)
         }
         __syncthreads();
     }
-    fft_run<E, +1>(v, t, a.P, a.tw, 1, ml);
+    F::template run<+1, 1>(v, t, a.P, a.tw, ml);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) col[(long long)(t + e * T) * L] = v[e];
     }
 }
 
-template <int E>
+template <class F>
 int launch_x(bool fwd, const pll_ppe_dev *d, const double *p_in, double *p_out, void *W,
              hipStream_t s)
 {
@@ -423,17 +522,17 @@ int launch_x(bool fwd, const pll_ppe_dev *d, const double *p_in, double *p_out, 
     const int threads = a.rows_per_wg * a.P.T;
     const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(cd);
-    if (threads > max_threads<E>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     if (fwd)
-        hipLaunchKernelGGL(k_xfwd<E>, dim3((unsigned)blocks), dim3(threads), shm, s, p_in,
+        hipLaunchKernelGGL(k_xfwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s, p_in,
                            (cd *)W, a);
     else
-        hipLaunchKernelGGL(k_xbwd<E>, dim3((unsigned)blocks), dim3(threads), shm, s,
+        hipLaunchKernelGGL(k_xbwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s,
                            (const cd *)W, p_out, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-template <int E>
+template <class F>
 int launch_y(bool fwd, const pll_ppe_dev *d, void *W, hipStream_t s)
 {
     YArgs a;
@@ -447,25 +546,47 @@ int launch_y(bool fwd, const pll_ppe_dev *d, void *W, hipStream_t s)
     const int threads = a.cols * a.P.T;
     const long long blocks = (long long)a.ntiles * d->nz;
     const size_t shm = (size_t)a.cols * a.pitch * sizeof(cd);
-    if (threads > max_threads<E>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     if (fwd)
-        hipLaunchKernelGGL(k_yfwd<E>, dim3((unsigned)blocks), dim3(threads), shm, s, (cd *)W, a);
+        hipLaunchKernelGGL(k_yfwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s, (cd *)W, a);
     else
-        hipLaunchKernelGGL(k_ybwd<E>, dim3((unsigned)blocks), dim3(threads), shm, s, (cd *)W, a);
+        hipLaunchKernelGGL(k_ybwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s, (cd *)W, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-template <typename F>
-int dispatch_E(int E, F &&f)
+// Compile-time plans for the transform lengths of the production grids
+// (x: L = nx/2, y: L = ny).  Every entry must match what ct_plan() reports.
+#define PLL_CT_PLANS(X)                 \
+    X(32, 8, 8, 4)                      \
+    X(48, 24, 8, 6)                     \
+    X(64, 8, 8, 8)                      \
+    X(96, 24, 8, 4, 3)                  \
+    X(128, 8, 8, 8, 2)                  \
+    X(192, 24, 8, 8, 3)                 \
+    X(192, 12, 4, 4, 4, 3)              \
+    X(256, 8, 8, 8, 4)                  \
+    X(384, 24, 8, 8, 6)                 \
+    X(384, 12, 4, 4, 4, 6)              \
+    X(512, 8, 8, 8, 8)                  \
+    X(768, 24, 8, 8, 4, 3)              \
+    X(768, 12, 4, 4, 4, 4, 3)           \
+    X(1024, 8, 8, 8, 8, 2)
+
+template <typename G>
+int dispatch_plan(const pll_fftplan &P, G &&g)
 {
-    switch (E) {
-        case 2: return f(std::integral_constant<int, 2>{});
-        case 3: return f(std::integral_constant<int, 3>{});
-        case 4: return f(std::integral_constant<int, 4>{});
-        case 6: return f(std::integral_constant<int, 6>{});
-        case 8: return f(std::integral_constant<int, 8>{});
-        case 12: return f(std::integral_constant<int, 12>{});
-        case 24: return f(std::integral_constant<int, 24>{});
+#define PLL_CT_CASE(L_, E_, ...) \
+    if (P.L == L_ && P.E == E_) return g(CtFFT<L_, E_, __VA_ARGS__>{});
+    PLL_CT_PLANS(PLL_CT_CASE)
+#undef PLL_CT_CASE
+    switch (P.E) {
+        case 2: return g(RtFFT<2>{});
+        case 3: return g(RtFFT<3>{});
+        case 4: return g(RtFFT<4>{});
+        case 6: return g(RtFFT<6>{});
+        case 8: return g(RtFFT<8>{});
+        case 12: return g(RtFFT<12>{});
+        case 24: return g(RtFFT<24>{});
         default: return -5;
     }
 }
@@ -479,24 +600,51 @@ extern "C" int pll_fft_supported_E(int E)
 
 extern "C" int pll_fft_max_threads(int E) { return E >= 12 ? 256 : 512; }
 
+extern "C" int pll_fft_ct_plan(int L, int prefer_E, pll_fftplan *p)
+{
+    const int plans[][8] = {
+#define PLL_CT_ROW(L_, E_, ...) {L_, E_, __VA_ARGS__},
+        PLL_CT_PLANS(PLL_CT_ROW)
+#undef PLL_CT_ROW
+    };
+    int pick = -1;
+    for (size_t i = 0; i < sizeof plans / sizeof plans[0]; i++) {
+        if (plans[i][0] != L) continue;
+        if (pick < 0 || plans[i][1] == prefer_E) pick = (int)i;
+    }
+    for (size_t i = (size_t)(pick < 0 ? 0 : pick); pick >= 0 && i == (size_t)pick; i++) {
+        p->L = L;
+        p->E = plans[i][1];
+        p->T = L / p->E;
+        int n = 0, prod = 1;
+        for (int j = 2; j < 8 && plans[i][j] > 0; j++) {
+            p->rad[n++] = plans[i][j];
+            prod *= plans[i][j];
+        }
+        p->nst = n;
+        return prod == L;
+    }
+    return 0;
+}
+
 extern "C" int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s)
 {
-    return dispatch_E(d->px.E, [&](auto e) { return launch_x<decltype(e)::value>(true, d, p, nullptr, W, s); });
+    return dispatch_plan(d->px, [&](auto f) { return launch_x<decltype(f)>(true, d, p, nullptr, W, s); });
 }
 
 extern "C" int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s)
 {
-    return dispatch_E(d->px.E, [&](auto e) {
-        return launch_x<decltype(e)::value>(false, d, nullptr, p, const_cast<void *>(W), s);
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_x<decltype(f)>(false, d, nullptr, p, const_cast<void *>(W), s);
     });
 }
 
 extern "C" int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
-    return dispatch_E(d->py.E, [&](auto e) { return launch_y<decltype(e)::value>(true, d, W, s); });
+    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f)>(true, d, W, s); });
 }
 
 extern "C" int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
-    return dispatch_E(d->py.E, [&](auto e) { return launch_y<decltype(e)::value>(false, d, W, s); });
+    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f)>(false, d, W, s); });
 }

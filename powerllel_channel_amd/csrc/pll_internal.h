@@ -43,6 +43,14 @@ typedef struct {
     const void *tw_y;          /* double2[ny]: exp(-2 pi i t / ny)  */
     const double *a, *c;       /* [nz] tridiagonal off-diagonals    */
     const double *piv;         /* [nz][ny/2+1][L+1] factorised diagonal */
+    struct {                   /* partitioned z solve (P = 0: streaming solve) */
+        int M, P, S;           /* rows per lane, lanes per column, columns per tile */
+        int qx, iyx;           /* tiles with q0 < qx and iy < iyx: exact path */
+        const double *bz;      /* [nz] diagonal before adding the eigenvalue */
+        const double *lamx;    /* [L+1]    */
+        const double *lamy;    /* [ny/2+1] */
+        const int *order;      /* [tiles] launch order, exact tiles first */
+    } tp;
 } pll_ppe_dev;
 
 int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s);
@@ -50,10 +58,15 @@ int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s);
+/* 1 if the partitioned z solve is instantiated for M rows per lane. */
+int pll_trid_part_supported(int M);
 /* 1 if a kernel instantiation exists for E (elements per thread). */
 int pll_fft_supported_E(int E);
 /* largest workgroup the E instantiation is compiled for */
 int pll_fft_max_threads(int E);
+/* compile-time plan for length L, if one exists (returns 1 and fills p);
+ * among several, the one with E == prefer_E, else the first listed */
+int pll_fft_ct_plan(int L, int prefer_E, pll_fftplan *p);
 
 /* Grid constants for the step kernels (device pointers for z arrays). */
 typedef struct {

@@ -47,6 +47,10 @@ int pll_set_error(int code, const char *fmt, ...)
 static int make_plan(int L, pll_fftplan *p)
 {
     memset(p, 0, sizeof *p);
+    /* PLL_FFT_E selects among compile-time plans of the same length (tuning) */
+    const char *pe = getenv("PLL_FFT_E");
+    if (pll_fft_ct_plan(L, pe ? atoi(pe) : 0, p)) return PLL_OK;
+    memset(p, 0, sizeof *p);
     int a2 = 0, b3 = 0, m = L;
     if (L < 2) return pll_set_error(PLL_EUNSUP, "FFT length %d < 2", L);
     while (m % 2 == 0) { m /= 2; a2++; }
@@ -127,15 +131,18 @@ static int trid_coeff(int n, const double *dzf, const char *bct, double *a, doub
 }
 
 /* ------------------------------------------------------------------ handle */
+#define PLL_TIMING_RING 128
 struct pll_ppe {
     int nx, ny, nz, L;
     pll_ppe_dev d;
     void *W, *tw_x, *tw_y, *a, *c, *piv;
+    void *bz, *lamx, *lamy, *order;   /* partitioned z solve */
     double *stage;   /* haloed staging buffer for host-pointer calls */
     size_t stage_bytes;
     size_t dev_bytes;
     int timing;
-    hipEvent_t ev[6];
+    int nrec;                             /* solves recorded since timing on */
+    hipEvent_t ev[PLL_TIMING_RING][6];    /* per-solve stage boundaries      */
 };
 
 static int dmalloc(pll_ppe *h, void **p, size_t bytes)
@@ -152,11 +159,13 @@ static int dmalloc(pll_ppe *h, void **p, size_t bytes)
 int pll_ppe_destroy(pll_ppe *h)
 {
     if (!h) return PLL_OK;
-    void *bufs[] = {h->W, h->tw_x, h->tw_y, h->a, h->c, h->piv, h->stage};
+    void *bufs[] = {h->W, h->tw_x, h->tw_y, h->a, h->c, h->piv, h->stage,
+                    h->bz, h->lamx, h->lamy, h->order};
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
         if (bufs[i]) hipFree(bufs[i]);
     if (h->timing)
-        for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[i]);
+        for (int r = 0; r < PLL_TIMING_RING; r++)
+            for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[r][i]);
     free(h);
     return PLL_OK;
 }
@@ -182,6 +191,36 @@ static int choose_launch(pll_ppe *h)
     h->d.y_cols_per_wg = cols;
     return PLL_OK;
 }
+
+/* Layout of the partitioned z solve: M rows per lane (M in {8,12,16,4}),
+ * P = nz/M lanes per column (a power of two, 2..64, so a column's lanes share
+ * one wavefront), S columns per tile (16, 8 or 4 doubles = one 128/64/32-byte
+ * row segment).  Returns 0 (streaming solve) if nothing fits. */
+static int choose_part(int nx, int nz, int *M, int *P, int *S)
+{
+    static const int Ms[] = {8, 12, 16, 4};
+    static const int Ss[] = {16, 8, 4};
+    for (size_t i = 0; i < sizeof Ms / sizeof Ms[0]; i++) {
+        const int m = Ms[i];
+        if (nz % m || !pll_trid_part_supported(m)) continue;
+        const int p = nz / m;
+        if (p < 2 || p > 64 || (p & (p - 1))) continue;
+        for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
+            const int sc = Ss[j];
+            if (nx % sc || sc * p > 512 || (size_t)nz * sc * 8 > 65536) continue;
+            *M = m;
+            *P = p;
+            *S = sc;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* Eigenvalue magnitude below which a column is solved with the sequential
+ * recurrence (relative to the largest diagonal of the z operator): the block
+ * decomposition's error grows like the condition number of the column. */
+#define PLL_EXACT_REL 2e-5
 
 int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
                    const double *dzf_global, const char *bctype_x, const char *bctype_y,
@@ -282,6 +321,66 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     h->d.a = (const double *)h->a;
     h->d.c = (const double *)h->c;
     h->d.piv = (const double *)h->piv;
+    {
+        int M, P, S;
+        if (!getenv("PLL_TRID_STREAMING") && choose_part(nx, nz, &M, &P, &S)) {
+            const int ntx = nx / S, ntiles = ntx * ny;
+            /* exact-path thresholds: |lambda| < tau * max|bz| */
+            double bmax = 0.0;
+            for (int k = 0; k < nz; k++) bmax = fabs(bz[k]) > bmax ? fabs(bz[k]) : bmax;
+            const double thr = PLL_EXACT_REL * bmax;
+            int iyx = 0;
+            while (iyx < nyh && fabs(ly[iyx]) < thr) iyx++;
+            int ixmax = -1;
+            while (ixmax + 1 <= L && fabs(lx[ixmax + 1]) < thr) ixmax++;
+            int qx = iyx > 0 ? 2 * (ixmax + 1) : 0;
+            qx = ((qx + S - 1) / S) * S;
+            int *order = (int *)malloc(sizeof(int) * (size_t)ntiles);
+            if (!order) {
+                rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
+                goto fail;
+            }
+            int n = 0;
+            for (int pass = 0; pass < 2; pass++)
+                for (int t = 0; t < ntiles; t++) {
+                    const int my = t / ntx, q0 = (t % ntx) * S;
+                    const int iy = my <= ny - my ? my : ny - my;
+                    const int ex = (q0 < qx && iy < iyx);
+                    if (ex == (pass == 0)) order[n++] = t;
+                }
+            double *lxh = (double *)malloc(sizeof(double) * (size_t)(L + 1));
+            if (!lxh) {
+                free(order);
+                rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
+                goto fail;
+            }
+            for (int i = 0; i <= L; i++) lxh[i] = lx[i];
+            if ((rc = dmalloc(h, &h->bz, sizeof(double) * (size_t)nz)) ||
+                (rc = dmalloc(h, &h->lamx, sizeof(double) * (size_t)(L + 1))) ||
+                (rc = dmalloc(h, &h->lamy, sizeof(double) * (size_t)nyh)) ||
+                (rc = dmalloc(h, &h->order, sizeof(int) * (size_t)ntiles)) ||
+                (hipMemcpy(h->bz, bz, sizeof(double) * (size_t)nz, hipMemcpyHostToDevice) != hipSuccess) ||
+                (hipMemcpy(h->lamx, lxh, sizeof(double) * (size_t)(L + 1), hipMemcpyHostToDevice) != hipSuccess) ||
+                (hipMemcpy(h->lamy, ly, sizeof(double) * (size_t)nyh, hipMemcpyHostToDevice) != hipSuccess) ||
+                (hipMemcpy(h->order, order, sizeof(int) * (size_t)ntiles, hipMemcpyHostToDevice) != hipSuccess)) {
+                free(order);
+                free(lxh);
+                if (!rc) rc = pll_set_error(PLL_EHIP, "copy of z-solve tables failed");
+                goto fail;
+            }
+            free(order);
+            free(lxh);
+            h->d.tp.M = M;
+            h->d.tp.P = P;
+            h->d.tp.S = S;
+            h->d.tp.qx = qx;
+            h->d.tp.iyx = iyx;
+            h->d.tp.bz = (const double *)h->bz;
+            h->d.tp.lamx = (const double *)h->lamx;
+            h->d.tp.lamy = (const double *)h->lamy;
+            h->d.tp.order = (const int *)h->order;
+        }
+    }
     free(a); free(c); free(bz); free(lx); free(ly); free(piv); free(twx); free(twy);
     *out = h;
     return PLL_OK;
@@ -297,21 +396,45 @@ int pll_ppe_set_timing(pll_ppe *h, int on)
 {
     if (!h) return pll_set_error(PLL_EINVAL, "NULL handle");
     if (on && !h->timing) {
-        for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&h->ev[i]));
+        for (int r = 0; r < PLL_TIMING_RING; r++)
+            for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&h->ev[r][i]));
         h->timing = 1;
     } else if (!on && h->timing) {
-        for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[i]);
+        for (int r = 0; r < PLL_TIMING_RING; r++)
+            for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[r][i]);
         h->timing = 0;
     }
+    h->nrec = 0;
     return PLL_OK;
 }
 
 int pll_ppe_stage_times(pll_ppe *h, float ms[5])
 {
     if (!h || !ms) return pll_set_error(PLL_EINVAL, "NULL argument");
-    if (!h->timing) return pll_set_error(PLL_ESTATE, "timing is off");
-    HIPCHK(hipEventSynchronize(h->ev[5]));
-    for (int i = 0; i < 5; i++) HIPCHK(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+    if (!h->timing || h->nrec == 0) return pll_set_error(PLL_ESTATE, "no timed solve recorded");
+    const int r = (h->nrec - 1) % PLL_TIMING_RING;
+    HIPCHK(hipEventSynchronize(h->ev[r][5]));
+    for (int i = 0; i < 5; i++) HIPCHK(hipEventElapsedTime(&ms[i], h->ev[r][i], h->ev[r][i + 1]));
+    return PLL_OK;
+}
+
+int pll_ppe_stage_times_mean(pll_ppe *h, double ms[5], int *nsolves)
+{
+    if (!h || !ms) return pll_set_error(PLL_EINVAL, "NULL argument");
+    if (!h->timing || h->nrec == 0) return pll_set_error(PLL_ESTATE, "no timed solve recorded");
+    const int n = h->nrec < PLL_TIMING_RING ? h->nrec : PLL_TIMING_RING;
+    for (int i = 0; i < 5; i++) ms[i] = 0.0;
+    for (int q = 0; q < n; q++) {
+        const int r = (h->nrec - 1 - q) % PLL_TIMING_RING;
+        HIPCHK(hipEventSynchronize(h->ev[r][5]));
+        for (int i = 0; i < 5; i++) {
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, h->ev[r][i], h->ev[r][i + 1]));
+            ms[i] += t;
+        }
+    }
+    for (int i = 0; i < 5; i++) ms[i] /= n;
+    if (nsolves) *nsolves = n;
     return PLL_OK;
 }
 
@@ -330,17 +453,21 @@ int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
     if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
     hipStream_t s = (hipStream_t)stream;
     int rc;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[0], s));
+    hipEvent_t *ev = h->timing ? h->ev[h->nrec % PLL_TIMING_RING] : NULL;
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
     if ((rc = launch_rc(pll_launch_xfwd(&h->d, d_var, h->W, s), "x forward"))) return rc;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[1], s));
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_rc(pll_launch_yfwd(&h->d, h->W, s), "y forward"))) return rc;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[2], s));
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if ((rc = launch_rc(pll_launch_trid(&h->d, h->W, s), "z solve"))) return rc;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[3], s));
+    if (ev) HIPCHK(hipEventRecord(ev[3], s));
     if ((rc = launch_rc(pll_launch_ybwd(&h->d, h->W, s), "y backward"))) return rc;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[4], s));
+    if (ev) HIPCHK(hipEventRecord(ev[4], s));
     if ((rc = launch_rc(pll_launch_xbwd(&h->d, h->W, d_var, s), "x backward"))) return rc;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[5], s));
+    if (ev) {
+        HIPCHK(hipEventRecord(ev[5], s));
+        h->nrec++;
+    }
     return PLL_OK;
 }
 

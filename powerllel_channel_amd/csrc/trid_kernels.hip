@@ -88,10 +88,241 @@ __global__ void __launch_bounds__(256) k_trid(cd *__restrict__ W, TArgs g)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Partitioned solve: one workgroup owns a tile of S consecutive real columns
+// (q0 .. q0+S-1 of one my-row, 8*S contiguous bytes per k) over all nz rows.
+// The tile is staged through LDS with coalesced row loads; then P = nz/M
+// lanes per column each hold M consecutive rows in registers:
+//   1. local elimination ("modified Thomas"): every row of the lane's block
+//      is rewritten as  a'_i x_first + x_i + c'_i x_last = d'_i, and the
+//      block's first/last rows couple only to the neighbouring blocks' last/
+//      first unknowns;
+//   2. the 2P interface unknowns (first, last of every block) form a
+//      tridiagonal system with unit diagonal, solved by parallel cyclic
+//      reduction across the P lanes with wavefront shuffles (the P lanes of
+//      a column sit in one wavefront);
+//   3. back substitution x_i = d'_i - a'_i x_first - c'_i x_last.
+// Every value is read from and written to HBM once; no pivot table.
+// Columns whose eigenvalue is close to the singular end of the spectrum are
+// more accurate with the sequential recurrence (the block decomposition loses
+// up to a few 1e-12 there), so tiles holding such columns run the reference's
+// Thomas recurrence with the precomputed pivots on the LDS-resident column
+// instead; the host lists them first in `order` so they start first.
+struct PArgs {
+    const double *a, *c, *bz, *lamx, *lamy, *piv;
+    const int *order;
+    int L, nx, ny, nz, P, S, ntx, pin_gauge;
+    int qx, iyx;  // exact path for tiles with q0 < qx and iy < iyx
+};
+
+__device__ __forceinline__ int swz(int k, int s, int S, int M)
+{
+    return k * S + (s ^ ((k / M) & (S - 1)));
+}
+
+__device__ __forceinline__ double shfl(double v, int src)
+{
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// eigenvalue column index of real column q (slot q/2, part q%2) in row my
+__device__ __forceinline__ int lam_ix(int q, int my, int L, int ny)
+{
+    const int mx = q >> 1, part = q & 1;
+    if (mx > 0) return mx;
+    if (my == 0 || 2 * my == ny) return part ? L : 0;
+    return (2 * my < ny) ? 0 : L;
+}
+
+template <int M>
+__global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArgs g)
+{
+    extern __shared__ double tl[];
+    const int S = g.S, P = g.P, nz = g.nz;
+    const int tid = threadIdx.x;
+    const int tile = g.order[blockIdx.x];
+    const int my = tile / g.ntx, q0 = (tile % g.ntx) * S;
+    const long long plane = (long long)g.nx * g.ny;
+    double *__restrict__ base = Wr + (long long)my * g.nx + q0;
+    const int nthr = S * P;
+
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const int e = tid + j * nthr;
+        const int k = e / S, s = e - k * S;
+        tl[swz(k, s, S, M)] = base[k * plane + s];
+    }
+    __syncthreads();
+
+    const int iy = my <= g.ny - my ? my : g.ny - my;
+    if (q0 < g.qx && iy < g.iyx) {
+        if (tid < S) {
+            const int s = tid, q = q0 + s;
+            const int ix = lam_ix(q, my, g.L, g.ny);
+            const long long pst = (long long)(g.L + 1) * (g.ny / 2 + 1);
+            const double *__restrict__ pv = g.piv + (long long)iy * (g.L + 1) + ix;
+            double prev = tl[swz(0, s, S, M)];
+            for (int k = 1; k < nz; ++k) {
+                const int o = swz(k, s, S, M);
+                double cur = tl[o];
+                cur -= (g.a[k] / pv[(k - 1) * pst]) * prev;
+                tl[o] = cur;
+                prev = cur;
+            }
+            const double bl = pv[(nz - 1) * pst];
+            const bool pin = g.pin_gauge && my == 0 && q == 0;
+            prev = pin ? 0.0 : (bl != 0.0 ? prev / bl : 0.0);
+            tl[swz(nz - 1, s, S, M)] = prev;
+            for (int k = nz - 2; k >= 0; --k) {
+                const int o = swz(k, s, S, M);
+                const double cur = (tl[o] - g.c[k] * prev) / pv[k * pst];
+                tl[o] = cur;
+                prev = cur;
+            }
+        }
+    } else {
+        const int s = tid / P, p = tid - s * P;
+        const int q = q0 + s;
+        const double lam = g.lamx[lam_ix(q, my, g.L, g.ny)] + g.lamy[iy];
+        const int k0 = p * M;
+        double d[M], ap[M], cp[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) d[i] = tl[swz(k0 + i, s, S, M)];
+        // 1. local elimination
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double r = 1.0 / (g.bz[k0 + i] + lam);
+            d[i] *= r;
+            ap[i] = g.a[k0 + i] * r;
+            cp[i] = g.c[k0 + i] * r;
+        }
+#pragma unroll
+        for (int i = 2; i < M; ++i) {
+            const double ai = g.a[k0 + i];
+            const double r = 1.0 / ((g.bz[k0 + i] + lam) - ai * cp[i - 1]);
+            d[i] = r * (d[i] - ai * d[i - 1]);
+            ap[i] = -r * ai * ap[i - 1];
+            cp[i] = r * g.c[k0 + i];
+        }
+#pragma unroll
+        for (int i = M - 3; i >= 1; --i) {
+            d[i] -= cp[i] * d[i + 1];
+            ap[i] -= cp[i] * ap[i + 1];
+            cp[i] = -cp[i] * cp[i + 1];
+        }
+        {
+            const double r = 1.0 / (1.0 - cp[0] * ap[1]);
+            d[0] = r * (d[0] - cp[0] * d[1]);
+            ap[0] = r * ap[0];
+            cp[0] = -r * cp[0] * cp[1];
+        }
+        // 2. reduced system: equation 2p = block's first row (F), 2p+1 = last (L)
+        double fa = ap[0], fg = cp[0], fd = d[0];
+        double la = ap[M - 1], lg = cp[M - 1], ld = d[M - 1];
+        if (g.pin_gauge && my == 0 && q == 0 && p == P - 1) {
+            la = 0.0; lg = 0.0; ld = 0.0;  // top value of the (0,0) mode = 0
+            fg = 0.0;
+        }
+        const int lane = tid & 63;
+        // stride 1: F_p <- L_{p-1}, L_p;  L_p <- F_p, F_{p+1}
+        {
+            const bool hasl = p > 0, hasr = p < P - 1;
+            const int sl = hasl ? lane - 1 : lane, sr = hasr ? lane + 1 : lane;
+            // every lane takes part in every shuffle; out-of-range neighbours
+            // are then replaced by the empty equation
+            const double ula = shfl(la, sl), ulg = shfl(lg, sl), uld = shfl(ld, sl);
+            const double dfa = shfl(fa, sr), dfg = shfl(fg, sr), dfd = shfl(fd, sr);
+            const double pla = hasl ? ula : 0.0, plg = hasl ? ulg : 0.0, pld = hasl ? uld : 0.0;
+            const double nfa = hasr ? dfa : 0.0, nfg = hasr ? dfg : 0.0, nfd = hasr ? dfd : 0.0;
+            // F: left = L_{p-1}, right = L_p
+            const double rf = 1.0 / (1.0 - fa * plg - fg * la);
+            const double fa2 = -fa * pla * rf, fg2 = -fg * lg * rf;
+            const double fd2 = (fd - fa * pld - fg * ld) * rf;
+            // L: left = F_p, right = F_{p+1}
+            const double rl = 1.0 / (1.0 - la * fg - lg * nfa);
+            const double la2 = -la * fa * rl, lg2 = -lg * nfg * rl;
+            const double ld2 = (ld - la * fd - lg * nfd) * rl;
+            fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
+        }
+        // stride 2t: F_p <- F_{p-t}, F_{p+t};  L_p <- L_{p-t}, L_{p+t}
+        for (int t = 1; t < P; t *= 2) {
+            const bool hasl = p - t >= 0, hasr = p + t < P;
+            const int sl = hasl ? lane - t : lane, sr = hasr ? lane + t : lane;
+            const double ufa = shfl(fa, sl), ufg = shfl(fg, sl), ufd = shfl(fd, sl);
+            const double dfa = shfl(fa, sr), dfg = shfl(fg, sr), dfd = shfl(fd, sr);
+            const double ula = shfl(la, sl), ulg = shfl(lg, sl), uld = shfl(ld, sl);
+            const double dla = shfl(la, sr), dlg = shfl(lg, sr), dld = shfl(ld, sr);
+            const double pfa = hasl ? ufa : 0.0, pfg = hasl ? ufg : 0.0, pfd = hasl ? ufd : 0.0;
+            const double nfa = hasr ? dfa : 0.0, nfg = hasr ? dfg : 0.0, nfd = hasr ? dfd : 0.0;
+            const double pla = hasl ? ula : 0.0, plg = hasl ? ulg : 0.0, pld = hasl ? uld : 0.0;
+            const double nla = hasr ? dla : 0.0, nlg = hasr ? dlg : 0.0, nld = hasr ? dld : 0.0;
+            const double rf = 1.0 / (1.0 - fa * pfg - fg * nfa);
+            const double fa2 = -fa * pfa * rf, fg2 = -fg * nfg * rf;
+            const double fd2 = (fd - fa * pfd - fg * nfd) * rf;
+            const double rl = 1.0 / (1.0 - la * plg - lg * nla);
+            const double la2 = -la * pla * rl, lg2 = -lg * nlg * rl;
+            const double ld2 = (ld - la * pld - lg * nld) * rl;
+            fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
+        }
+        // 3. back substitution
+        const double F = fd, Lv = ld;
+#pragma unroll
+        for (int i = 1; i < M - 1; ++i) d[i] = d[i] - ap[i] * F - cp[i] * Lv;
+        d[0] = F;
+        d[M - 1] = Lv;
+#pragma unroll
+        for (int i = 0; i < M; ++i) tl[swz(k0 + i, s, S, M)] = d[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const int e = tid + j * nthr;
+        const int k = e / S, s = e - k * S;
+        base[k * plane + s] = tl[swz(k, s, S, M)];
+    }
+}
+
 }  // namespace
+
+extern "C" int pll_trid_part_supported(int M) { return M == 4 || M == 8 || M == 12 || M == 16; }
 
 extern "C" int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
+    if (d->tp.P > 0) {
+        PArgs g;
+        g.a = d->a;
+        g.c = d->c;
+        g.bz = d->tp.bz;
+        g.lamx = d->tp.lamx;
+        g.lamy = d->tp.lamy;
+        g.piv = d->piv;
+        g.order = d->tp.order;
+        g.L = d->L;
+        g.nx = d->nx;
+        g.ny = d->ny;
+        g.nz = d->nz;
+        g.P = d->tp.P;
+        g.S = d->tp.S;
+        g.ntx = d->nx / d->tp.S;
+        g.pin_gauge = d->pin_gauge;
+        g.qx = d->tp.qx;
+        g.iyx = d->tp.iyx;
+        const unsigned blocks = (unsigned)(g.ntx * d->ny);
+        const int threads = g.S * g.P;
+        const size_t shm = sizeof(double) * (size_t)d->nz * g.S;
+        if (threads > 512 || shm > 65536) return -1;
+        switch (d->tp.M) {
+            case 4: hipLaunchKernelGGL(k_trid_part<4>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
+            case 8: hipLaunchKernelGGL(k_trid_part<8>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
+            case 12: hipLaunchKernelGGL(k_trid_part<12>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
+            case 16: hipLaunchKernelGGL(k_trid_part<16>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
+            default: return -5;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : -3;
+    }
     TArgs g;
     g.a = d->a;
     g.c = d->c;

@@ -92,6 +92,13 @@ class PoissonSolver:
         check(lib().pll_ppe_stage_times(self._h, out))
         return list(out)
 
+    def stage_times_mean(self):
+        """(mean ms per stage over the timed solves since set_timing(True), count)."""
+        out = (C.c_double * 5)()
+        n = C.c_int()
+        check(lib().pll_ppe_stage_times_mean(self._h, out, C.byref(n)))
+        return list(out), n.value
+
     def device_bytes(self) -> int:
         return int(lib().pll_ppe_device_bytes(self._h))
 

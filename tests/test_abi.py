@@ -1,0 +1,103 @@
+"""The C ABI boundary without a GPU: the library loads, exports every entry
+point include/powerllel.h declares, and rejects bad arguments before any
+device call (the reference would abort on them)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "powerllel.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*([a-z_][a-z0-9_]*)\s*\(", txt, re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from powerllel_channel_amd import lib as L
+    return L()
+
+
+def test_header_parsed():
+    names = _declared()
+    for n in ("init_poisson_solver", "execute_poisson_solver", "free_poisson_solver",
+              "pll_ppe_create", "pll_channel_step"):
+        assert n in names
+
+
+def test_every_declared_symbol_exported(lib):
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_bindings_cover_header():
+    from powerllel_channel_amd._lib import exported_symbols
+    assert set(_declared()) == set(exported_symbols())
+
+
+def _create(lib, nx, ny, nz, bcx=b"PP", bcy=b"PP", bcz=b"NN", dzf=None):
+    h = C.c_void_p()
+    dzf = np.full(nz + 2, 2.0 / nz) if dzf is None else dzf
+    rc = lib.pll_ppe_create(C.byref(h), nx, ny, nz, 0.1, 0.1, dzf.ctypes.data, bcx, bcy, bcz)
+    return rc, h
+
+
+@pytest.mark.parametrize("args,code", [
+    ((7, 8, 8), -1),              # odd nx
+    ((8, 7, 8), -1),              # odd ny
+    ((8, 8, 1), -1),              # nz < 2
+    ((10, 8, 8), -5),             # nx/2 = 5: radix 5 not implemented
+])
+def test_create_rejects_bad_sizes(lib, args, code):
+    rc, h = _create(lib, *args)
+    assert rc == code and not h.value
+    assert lib.pll_last_error()
+
+
+@pytest.mark.parametrize("bcx,bcy,bcz", [(b"NN", b"PP", b"NN"), (b"PP", b"DD", b"NN"),
+                                          (b"PP", b"PP", b"PP"), (b"PP", b"PP", b"N")])
+def test_create_rejects_bc(lib, bcx, bcy, bcz):
+    rc, h = _create(lib, 8, 8, 8, bcx, bcy, bcz)
+    assert rc in (-1, -5) and not h.value
+
+
+def test_create_rejects_bad_dzf(lib):
+    dzf = np.full(10, 0.25)
+    dzf[4] = 0.0
+    rc, h = _create(lib, 8, 8, 8, dzf=dzf)
+    assert rc == -1
+
+
+def test_dropin_order_and_neighbours(lib):
+    assert lib.execute_poisson_solver_status(None) == -4   # before init
+    nb = np.zeros((3, 6), dtype=np.int32)                   # z-pencil split: unsupported
+    dzf = np.full(10, 0.25)
+    rc = lib.init_poisson_solver_status(8, 8, 8, 0.1, 0.1, dzf.ctypes.data, b"PP", b"PP", b"NN",
+                                        nb.ctypes.data)
+    assert rc == -5 and b"MPI_PROC_NULL" in lib.pll_last_error()
+    assert lib.free_poisson_solver_status() == 0
+
+
+def test_channel_rejects_bad_grid(lib):
+    h = C.c_void_p()
+    dzf = np.full(6, 0.5)
+    isf = (C.c_int * 3)(1, 0, 0)
+    vf = (C.c_double * 3)(1.0, 0.0, 0.0)
+    rc = lib.pll_channel_create(C.byref(h), 8, 8, 4, 1.0, 1.0, 2.0, dzf.ctypes.data, 1e-3, -1.0,
+                                0.0, isf, vf)
+    assert rc == -1 and not h.value
+
+
+def test_bench_refuses_multi_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr

@@ -1,0 +1,158 @@
+"""Pins of the oracle's step routines (restating src/mod_calcVel.f90,
+src/mod_calcRHS.f90, src/mod_updateBound.f90, src/mod_utils.f90) by
+properties the discretisation fixes, not by retyping its formulas."""
+import math
+
+import numpy as np
+import pytest
+
+LX, LY, LZ = 4 * math.pi, 2 * math.pi, 2.0
+
+
+def _grid(orc, nx, ny, nz, sr, nu=1 / 5600, dt=1e-3):
+    dzf = orc.mesh_tanh(nz, LZ, sr)
+    return orc.OracleGrid(nx, ny, nz, LX, LY, LZ, dzf, nu, dt)
+
+
+def test_rk1_pure_viscous_profile(orc):
+    """u = u(z), v = w = 0: advection vanishes and RK1 is u + dt nu D2z u with
+    D2z the z operator of the pressure solver (set_trid_coeff 'c', interior
+    rows), an independent code path."""
+    nx, ny, nz = 8, 6, 16
+    G = _grid(orc, nx, ny, nz, 3.0)
+    prof = np.random.default_rng(1).standard_normal(nz + 2)
+    u = np.broadcast_to(prof[:, None, None], G.shape).copy()
+    v, w = G.zeros(), G.zeros()
+    u1, v1, w1 = G.zeros(), G.zeros(), G.zeros()
+    orc.rk1(G, u, v, w, u1, v1, w1)
+    a, b, c = orc.trid_coeff(G.dzf, "NN", walls=False)  # no wall correction
+    d2 = a * prof[:-2] + b * prof[1:-1] + c * prof[2:]
+    expect = prof[1:-1] + G.dt * G.nu * d2
+    got = u1[1:-1, 3, 4]
+    assert np.allclose(got, expect, rtol=0, atol=1e-13 * np.abs(prof).max())
+    assert np.abs(v1[1:-1, 1:-1, 1:-1]).max() == 0.0
+    assert np.abs(w1[1:-1, 1:-1, 1:-1]).max() == 0.0
+
+
+def test_rk1_constant_field(orc):
+    """A uniform (U, V, W) field (ghosts included) is a fixed point."""
+    G = _grid(orc, 12, 10, 8, 4.0)
+    u = np.full(G.shape, 0.7); v = np.full(G.shape, -0.3); w = np.full(G.shape, 0.2)
+    u1, v1, w1 = G.zeros(), G.zeros(), G.zeros()
+    orc.rk1(G, u, v, w, u1, v1, w1, 0.0)
+    for q, q1 in ((u, u1), (v, v1), (w, w1)):
+        assert np.abs(q1[1:-1, 1:-1, 1:-1] - q[1:-1, 1:-1, 1:-1]).max() < 1e-15
+
+
+@pytest.mark.parametrize("u_crf", [0.0, 0.45])
+def test_advection_conserves_momentum(orc, u_crf):
+    """With nu = 0 the divergence-form advection telescopes: over the periodic
+    box with impermeable walls, sum of (u1-u)*dzf and of (v1-v)*dzf is zero."""
+    nx, ny, nz = 16, 12, 10
+    G = _grid(orc, nx, ny, nz, 3.0, nu=0.0, dt=1e-2)
+    rng = np.random.default_rng(3)
+    u, v, w = (rng.standard_normal(G.shape) for _ in range(3))
+    orc.bc_vel(G, u, v, w, u_crf)
+    u1, v1, w1 = G.zeros(), G.zeros(), G.zeros()
+    orc.rk1(G, u, v, w, u1, v1, w1, u_crf)
+    wz = G.dzf[1:-1][:, None, None]
+    for q, q1 in ((u, u1), (v, v1)):
+        dm = ((q1 - q)[1:-1, 1:-1, 1:-1] * wz).sum()
+        scale = (np.abs(q)[1:-1, 1:-1, 1:-1] * wz).sum() * G.dt
+        assert abs(dm) < 1e-13 * scale
+
+
+def test_rk2_is_heun(orc):
+    """RK2 from (u, u1) equals (u + u1 + dt F(u1)) / 2 with F taken from an RK1
+    evaluation at u1 (the reference's Heun scheme, mod_calcVel.f90:282)."""
+    G = _grid(orc, 10, 8, 6, 3.0)
+    rng = np.random.default_rng(4)
+    u, v, w = (rng.standard_normal(G.shape) for _ in range(3))
+    u1, v1, w1 = (rng.standard_normal(G.shape) for _ in range(3))
+    u2, v2, w2 = G.zeros(), G.zeros(), G.zeros()
+    orc.rk1(G, u1, v1, w1, u2, v2, w2)          # u2 = u1 + dt F(u1)
+    uu, vv, ww = u.copy(), v.copy(), w.copy()
+    orc.rk2(G, uu, vv, ww, u1, v1, w1)
+    inner = (slice(1, -1),) * 3
+    for got, q, q2 in ((uu, u, u2), (vv, v, v2), (ww, w, w2)):
+        assert np.allclose(got[inner], 0.5 * (q[inner] + q2[inner]), rtol=0, atol=1e-14)
+
+
+def test_bc_vel_semantics(orc):
+    """Periodic ghosts in x and y (corners included), no-slip walls: the
+    interpolated wall value of u is -u_crf, of v is 0, w = 0 on both walls."""
+    nx, ny, nz, u_crf = 6, 5, 4, 0.3
+    G = _grid(orc, nx, ny, nz, 3.0)
+    rng = np.random.default_rng(7)
+    u, v, w = (rng.standard_normal(G.shape) for _ in range(3))
+    orc.bc_vel(G, u, v, w, u_crf)
+    for q in (u, v):
+        core = q[1:-1]
+        assert np.array_equal(core[:, :, 0], core[:, :, nx])
+        assert np.array_equal(core[:, :, nx + 1], core[:, :, 1])
+        assert np.array_equal(core[:, 0, :], core[:, ny, :])
+        assert np.array_equal(core[:, ny + 1, :], core[:, 1, :])
+    assert np.allclose(0.5 * (u[0] + u[1]), -u_crf, atol=1e-15)
+    assert np.allclose(0.5 * (u[-1] + u[-2]), -u_crf, atol=1e-15)
+    assert np.allclose(0.5 * (v[0] + v[1]), 0.0, atol=1e-15)
+    assert np.allclose(0.5 * (v[-1] + v[-2]), 0.0, atol=1e-15)
+    assert not w[0].any() and not w[nz].any() and not w[nz + 1].any()
+
+
+def test_bc_p_zero_gradient(orc):
+    G = _grid(orc, 6, 4, 5, 3.0)
+    p = np.random.default_rng(8).standard_normal(G.shape)
+    orc.bc_p(G, p)
+    assert np.array_equal(p[0], p[1]) and np.array_equal(p[-1], p[-2])
+    assert np.array_equal(p[:, :, 0], p[:, :, -2]) and np.array_equal(p[:, 0, :], p[:, -2, :])
+    assert np.array_equal(p[:, :, -1], p[:, :, 1]) and np.array_equal(p[:, -1, :], p[:, 1, :])
+
+
+def test_projection_is_divergence_free(orc):
+    """calcRHS -> PPE -> updateBoundP -> correctVel leaves max|div u| at
+    roundoff (the reference's div_limit check, param.in:33)."""
+    nx, ny, nz = 24, 16, 20
+    G = _grid(orc, nx, ny, nz, 4.0, dt=2e-3)
+    rng = np.random.default_rng(11)
+    u, v, w = (rng.standard_normal(G.shape) for _ in range(3))
+    orc.bc_vel(G, u, v, w, 0.0)
+    assert orc.max_div(G, u, v, w)[0] > 1.0
+    p = G.zeros()
+    orc.calc_rhs(G, u, v, w, p)
+    orc.OraclePPE(nx, ny, nz, G.dx, G.dy, G.dzf, pin_gauge=True).solve(p)
+    orc.bc_p(G, p)
+    orc.correct_vel(G, p, u, v, w)
+    orc.bc_vel(G, u, v, w, 0.0)
+    div, _ = orc.max_div(G, u, v, w)
+    assert div < 1e-11
+
+
+def test_force_vel_sets_bulk_velocity(orc):
+    G = _grid(orc, 16, 8, 12, 3.0)
+    u, v, w = (np.random.default_rng(12).standard_normal(G.shape) for _ in range(3))
+    m0 = orc.mean_h(G, u)
+    wz = G.dzf[1:-1][:, None, None]
+    assert abs(m0 - (u[1:-1, 1:-1, 1:-1] * wz).sum() / (G.lz * G.nx * G.ny)) < 1e-14
+    orc.force_vel(G, u, v, w, (True, False, False), (1.0, 0.0, 0.0))
+    assert abs(orc.mean_h(G, u) - 1.0) < 1e-14
+
+
+def test_max_div_planted(orc):
+    G = _grid(orc, 8, 6, 5, 3.0)
+    u, v, w = G.zeros(), G.zeros(), G.zeros()
+    u[3, 2, 5] = 1.0  # div at (i=5,j=2,k=3) = +1/dx and at i=6: -1/dx
+    val, ijk = orc.max_div(G, u, v, w)
+    assert abs(val - 1.0 / G.dx) < 1e-14 / G.dx and ijk == (5, 2, 3)
+
+
+def test_wall_shear_laminar(orc):
+    """Poiseuille u = 1.5(1-z^2): tau_w -> 3 nu as dz -> 0 (one-sided estimate
+    of mod_monitor.f90:243, first order in the wall cell height)."""
+    from powerllel_channel_amd.mesh import cell_centres
+    errs = []
+    for nz in (32, 64, 128):
+        G = _grid(orc, 4, 4, nz, 3.0, nu=0.01)
+        zc = cell_centres(G.dzf) - 1.0
+        u = np.broadcast_to((1.5 * (1 - zc ** 2))[:, None, None], G.shape).copy()
+        errs.append(abs(orc.wall_shear(G, u) - 3 * 0.01))
+    assert errs[2] < errs[1] < errs[0] and errs[2] < 2e-3
