@@ -92,6 +92,52 @@ def bench_ppe(torch, dev, steps, warmup):
     return ms, dict(zip(STAGES, stage_ms)), nrec, dev_bytes
 
 
+# refined form, algorithmic bytes per cell: FP32 solve 48 (x_fwd reads the FP64
+# rhs 8 + writes W32 4; y, z, y read + write 4 each; x_bwd reads 4 + writes
+# FP64 8); each sweep 72 (residual reads f 8 + P 8 writes r 4; x_fwd 8; y, z, y
+# 24; x_bwd reads W32 4 + P 8 writes 8).
+def refined_bytes_per_cell(k):
+    return 48 + 72 * k
+
+
+def bench_refined(torch, dev, steps, warmup, sweeps=(0, 1, 2)):
+    """Refined form at the headline grid: ms per solve and the relative
+    max-norm difference to the FP64 solve, per sweep count."""
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, spectral_rhs
+    g = Grid(NX, NY, NZ, LX, LY, LZ, 2.0 * GAMMA)
+    rhs = torch.from_numpy(spectral_rhs(g, 0)).to(dev)
+    s = PoissonSolver(NX, NY, NZ, g.dx, g.dy, g.dzf)
+    ref = rhs.clone()
+    s.solve(ref)
+    inner = (slice(1, -1),) * 3
+    scale = float(ref[inner].abs().max())
+    out = {}
+    for k in sweeps:
+        p = rhs.clone()
+        s.solve_refined(p, k)
+        torch.cuda.synchronize()
+        err = float((p[inner] - ref[inner]).abs().max()) / scale
+        for _ in range(warmup):
+            s.solve_refined(p, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.solve_refined(p, k)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        gbs = refined_bytes_per_cell(k) * NX * NY * NZ / (ms * 1e-3) / 1e9
+        log(f"[bench] refined k={k}: {ms:.3f} ms/solve, rel err vs FP64 {err:.2e}")
+        out[str(k)] = {"ms": round(ms, 4), "solves_per_s": round(1e3 / ms, 2),
+                       "rel_err_vs_fp64": err, "bytes_per_cell": refined_bytes_per_cell(k),
+                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        del p
+    s.close()
+    del rhs, ref
+    return out
+
+
 def bench_step(torch, dev, steps, warmup):
     from powerllel_channel_amd import Channel, PoissonSolver
     from powerllel_channel_amd.mesh import Grid
@@ -235,6 +281,10 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "ppe_device_bytes": dev_bytes,
     }
+    refined = bench_refined(torch, dev, args.steps, args.warmup)
+    for v in refined.values():
+        v["speedup_vs_fp64"] = round(ms / v["ms"], 3)
+    out["refined"] = refined
     if not args.ppe_only:
         sms, div = bench_step(torch, dev, args.steps, args.warmup)
         step_gbs = STEP_BYTES_PER_CELL * N / (sms * 1e-3) / 1e9

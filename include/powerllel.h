@@ -97,6 +97,18 @@ int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream);
 int pll_ppe_set_timing(pll_ppe *h, int on);
 int pll_ppe_stage_times(pll_ppe *h, float ms[5]);
 int pll_ppe_stage_times_mean(pll_ppe *h, double ms[5], int *nsolves);
+/* Refined form (no counterpart in the reference, whose solver is FP64
+ * only): the transforms, the spectral array and the z solves run in FP32
+ * storage (z recurrence in FP64 registers); then `sweeps` rounds of
+ * r = f - L_h p computed in FP64, an FP32 solve L c = r and p += c.  The
+ * operator, gauge and in-place semantics are those of pll_ppe_solve.  The
+ * first call allocates 2 N floats + N doubles of work space. */
+int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream);
+/* Make pll_ppe_solve (and so the drop-in execute_poisson_solver) use the
+ * refined form with `sweeps` sweeps; -1 restores the FP64 solve. */
+int pll_ppe_set_refinement(pll_ppe *h, int sweeps);
+/* Same switch for the drop-in global instance (after init_poisson_solver). */
+int set_poisson_solver_refinement(int sweeps);
 /* Device bytes held by the handle. */
 size_t pll_ppe_device_bytes(const pll_ppe *h);
 int pll_ppe_destroy(pll_ppe *h);

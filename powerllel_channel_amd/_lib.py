@@ -46,6 +46,9 @@ _SIGS = {
     "pll_ppe_set_timing": (_ip, [_vp, _ip]),
     "pll_ppe_stage_times": (_ip, [_vp, C.POINTER(C.c_float)]),
     "pll_ppe_stage_times_mean": (_ip, [_vp, C.POINTER(_d), C.POINTER(_ip)]),
+    "pll_ppe_solve_refined": (_ip, [_vp, _dp, _ip, _vp]),
+    "pll_ppe_set_refinement": (_ip, [_vp, _ip]),
+    "set_poisson_solver_refinement": (_ip, [_ip]),
     "pll_ppe_device_bytes": (C.c_size_t, [_vp]),
     "pll_ppe_destroy": (_ip, [_vp]),
     "pll_channel_create": (_ip, [C.POINTER(_vp), _ip, _ip, _ip, _d, _d, _d, _dp, _d, _d, _d,

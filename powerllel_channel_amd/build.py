@@ -26,6 +26,7 @@ ARCH = "gfx950"
 HIP_SOURCES = {
     "fft_kernels.hip": [],
     "trid_kernels.hip": [],
+    "resid_kernels.hip": [],
     # restates the reference's Fortran arithmetic: no fused multiply-add
     "step_kernels.hip": ["-ffp-contract=off"],
 }

@@ -15,28 +15,55 @@
 // complex FFT down columns of that array; column 0 (two real spectra packed
 // in one complex column) is split into its two Hermitian halves.
 //
-// All tables (twiddles) come from the host, computed in long double.
+// Every kernel exists in double (the FP64 solve) and float (the FP32 solve
+// of the refined form: FFT arithmetic and the spectral array in FP32, the
+// real-space input and output in FP64).  Twiddle tables come from the host,
+// computed in long double and rounded once to the working precision.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "pll_internal.h"
 
 namespace {
 
-typedef double2 cd;
+template <typename V>
+struct Sc;
+template <>
+struct Sc<double2> {
+    using T = double;
+};
+template <>
+struct Sc<float2> {
+    using T = float;
+};
 
-__device__ __forceinline__ cd cadd(cd a, cd b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ cd csub(cd a, cd b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ cd cscale(cd a, double s) { return make_double2(a.x * s, a.y * s); }
-__device__ __forceinline__ cd cconj(cd a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ cd cmul(cd a, cd b)
+template <typename V>
+__device__ __forceinline__ V mk(typename Sc<V>::T x, typename Sc<V>::T y)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    V r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
+template <typename V>
+__device__ __forceinline__ V cscale(V a, typename Sc<V>::T s) { return mk<V>(a.x * s, a.y * s); }
+template <typename V>
+__device__ __forceinline__ V cconj(V a) { return mk<V>(a.x, -a.y); }
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b)
+{
+    return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // DIR * i * a
-template <int DIR>
-__device__ __forceinline__ cd rot(cd a)
+template <int DIR, typename V>
+__device__ __forceinline__ V rot(V a)
 {
-    return DIR < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+    return DIR < 0 ? mk<V>(a.y, -a.x) : mk<V>(-a.y, a.x);
 }
 
 // Two doubles at an 8-byte (not 16-byte) aligned address: the interior of a
@@ -53,9 +80,10 @@ struct Dft;
 
 template <int DIR>
 struct Dft<2, DIR> {
-    static __device__ __forceinline__ void run(cd *x)
+    template <typename V>
+    static __device__ __forceinline__ void run(V *x)
     {
-        cd a = x[0];
+        V a = x[0];
         x[0] = cadd(a, x[1]);
         x[1] = csub(a, x[1]);
     }
@@ -63,12 +91,14 @@ struct Dft<2, DIR> {
 
 template <int DIR>
 struct Dft<3, DIR> {
-    static __device__ __forceinline__ void run(cd *x)
+    template <typename V>
+    static __device__ __forceinline__ void run(V *x)
     {
-        cd s = cadd(x[1], x[2]);
-        cd d = csub(x[1], x[2]);
-        cd m = csub(x[0], cscale(s, 0.5));
-        cd r = rot<DIR>(cscale(d, kS3));
+        using S = typename Sc<V>::T;
+        V s = cadd(x[1], x[2]);
+        V d = csub(x[1], x[2]);
+        V m = csub(x[0], cscale(s, S(0.5)));
+        V r = rot<DIR>(cscale(d, S(kS3)));
         x[0] = cadd(x[0], s);
         x[1] = cadd(m, r);
         x[2] = csub(m, r);
@@ -77,12 +107,13 @@ struct Dft<3, DIR> {
 
 template <int DIR>
 struct Dft<4, DIR> {
-    static __device__ __forceinline__ void run(cd *x)
+    template <typename V>
+    static __device__ __forceinline__ void run(V *x)
     {
-        cd t0 = cadd(x[0], x[2]);
-        cd t1 = csub(x[0], x[2]);
-        cd t2 = cadd(x[1], x[3]);
-        cd t3 = rot<DIR>(csub(x[1], x[3]));
+        V t0 = cadd(x[0], x[2]);
+        V t1 = csub(x[0], x[2]);
+        V t2 = cadd(x[1], x[3]);
+        V t3 = rot<DIR>(csub(x[1], x[3]));
         x[0] = cadd(t0, t2);
         x[2] = csub(t0, t2);
         x[1] = cadd(t1, t3);
@@ -92,15 +123,17 @@ struct Dft<4, DIR> {
 
 template <int DIR>
 struct Dft<6, DIR> {
-    static __device__ __forceinline__ void run(cd *x)
+    template <typename V>
+    static __device__ __forceinline__ void run(V *x)
     {
-        cd a[3] = {x[0], x[2], x[4]};
-        cd b[3] = {x[1], x[3], x[5]};
+        using S = typename Sc<V>::T;
+        V a[3] = {x[0], x[2], x[4]};
+        V b[3] = {x[1], x[3], x[5]};
         Dft<3, DIR>::run(a);
         Dft<3, DIR>::run(b);
         // W6^1 b1 = b1/2 + S3*rot(b1);  W6^2 b2 = -b2/2 + S3*rot(b2)
-        cd b1 = cadd(cscale(b[1], 0.5), cscale(rot<DIR>(b[1]), kS3));
-        cd b2 = csub(cscale(rot<DIR>(b[2]), kS3), cscale(b[2], 0.5));
+        V b1 = cadd(cscale(b[1], S(0.5)), cscale(rot<DIR>(b[1]), S(kS3)));
+        V b2 = csub(cscale(rot<DIR>(b[2]), S(kS3)), cscale(b[2], S(0.5)));
         x[0] = cadd(a[0], b[0]);
         x[3] = csub(a[0], b[0]);
         x[1] = cadd(a[1], b1);
@@ -112,15 +145,17 @@ struct Dft<6, DIR> {
 
 template <int DIR>
 struct Dft<8, DIR> {
-    static __device__ __forceinline__ void run(cd *x)
+    template <typename V>
+    static __device__ __forceinline__ void run(V *x)
     {
-        cd e[4] = {x[0], x[2], x[4], x[6]};
-        cd o[4] = {x[1], x[3], x[5], x[7]};
+        using S = typename Sc<V>::T;
+        V e[4] = {x[0], x[2], x[4], x[6]};
+        V o[4] = {x[1], x[3], x[5], x[7]};
         Dft<4, DIR>::run(e);
         Dft<4, DIR>::run(o);
-        cd o1 = cscale(cadd(o[1], rot<DIR>(o[1])), kH);
-        cd o2 = rot<DIR>(o[2]);
-        cd o3 = cscale(csub(rot<DIR>(o[3]), o[3]), kH);
+        V o1 = cscale(cadd(o[1], rot<DIR>(o[1])), S(kH));
+        V o2 = rot<DIR>(o[2]);
+        V o3 = cscale(csub(rot<DIR>(o[3]), o[3]), S(kH));
         x[0] = cadd(e[0], o[0]);
         x[4] = csub(e[0], o[0]);
         x[1] = cadd(e[1], o1);
@@ -137,9 +172,9 @@ struct Dft<8, DIR> {
 // twiddles them by w^(r*k) (k = b mod Ns, w = exp(DIR 2 pi i/(Ns R))) and
 // writes output r to position (b - k)*R + k + r*Ns.  In the last stage that
 // position is element t + (q + r*E/R)*T, i.e. the same register slot.
-template <int R, int E, int DIR>
-__device__ __forceinline__ void stage(cd (&v)[E], int t, int T, int Ns, int twstep,
-                                      const cd *__restrict__ tw, cd *lds, bool last)
+template <int R, int E, int DIR, typename V>
+__device__ __forceinline__ void stage(V (&v)[E], int t, int T, int Ns, int twstep,
+                                      const V *__restrict__ tw, V *lds, bool last)
 {
     if constexpr (E % R == 0) {
         constexpr int Q = E / R;
@@ -147,13 +182,13 @@ __device__ __forceinline__ void stage(cd (&v)[E], int t, int T, int Ns, int twst
         for (int q = 0; q < Q; ++q) {
             const int b = t + q * T;
             const int k = b % Ns;
-            cd x[R];
+            V x[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    cd w = tw[r * k * twstep];
+                    V w = tw[r * k * twstep];
                     if (DIR > 0) w.y = -w.y;
                     x[r] = cmul(x[r], w);
                 }
@@ -171,13 +206,13 @@ __device__ __forceinline__ void stage(cd (&v)[E], int t, int T, int Ns, int twst
     }
 }
 
-// Full transform.  On entry v holds elements t + e*T of the input; on exit
-// it holds elements t + e*T of the output.  `lds` is this transform's LDS
-// segment (>= L points).  twscale = (table length) / L.  Every thread of the
-// workgroup must call it (it contains barriers).
-template <int E, int DIR>
-__device__ __forceinline__ void fft_run(cd (&v)[E], int t, const pll_fftplan &P,
-                                        const cd *__restrict__ tw, int twscale, cd *lds)
+// Full transform with a runtime plan.  On entry v holds elements t + e*T of
+// the input; on exit it holds elements t + e*T of the output.  `lds` is this
+// transform's LDS segment (>= L points).  twscale = (table length) / L.
+// Every thread of the workgroup must call it (it contains barriers).
+template <int E, int DIR, typename V>
+__device__ __forceinline__ void fft_run(V (&v)[E], int t, const pll_fftplan &P,
+                                        const V *__restrict__ tw, int twscale, V *lds)
 {
     const int T = P.T, L = P.L;
     int Ns = 1;
@@ -206,21 +241,21 @@ __device__ __forceinline__ void fft_run(cd (&v)[E], int t, const pll_fftplan &P,
 // Compile-time form of the same stages: radices, Ns and T are constants, so
 // the butterfly indices (b mod Ns, output positions, twiddle offsets) fold
 // into shifts and small constants and nothing is spilled between radices.
-template <int R, int E, int DIR, int T, int Ns, int TWSTEP, bool LAST>
-__device__ __forceinline__ void stage_ct(cd (&v)[E], int t, const cd *__restrict__ tw, cd *lds)
+template <int R, int E, int DIR, int T, int Ns, int TWSTEP, bool LAST, typename V>
+__device__ __forceinline__ void stage_ct(V (&v)[E], int t, const V *__restrict__ tw, V *lds)
 {
     constexpr int Q = E / R;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int b = t + q * T;
         const int k = b % Ns;
-        cd x[R];
+        V x[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cd w = tw[r * k * TWSTEP];
+                V w = tw[r * k * TWSTEP];
                 if (DIR > 0) w.y = -w.y;
                 x[r] = cmul(x[r], w);
             }
@@ -237,8 +272,8 @@ __device__ __forceinline__ void stage_ct(cd (&v)[E], int t, const cd *__restrict
     }
 }
 
-template <int L, int E, int DIR, int TWSCALE, int Ns, int R, int... Rest>
-__device__ __forceinline__ void stages_ct(cd (&v)[E], int t, const cd *__restrict__ tw, cd *lds)
+template <typename V, int L, int E, int DIR, int TWSCALE, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stages_ct(V (&v)[E], int t, const V *__restrict__ tw, V *lds)
 {
     static_assert(E % R == 0, "radix must divide E");
     constexpr int T = L / E;
@@ -250,7 +285,7 @@ __device__ __forceinline__ void stages_ct(cd (&v)[E], int t, const cd *__restric
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = lds[t + e * T];
         __syncthreads();
-        stages_ct<L, E, DIR, TWSCALE, Ns * R, Rest...>(v, t, tw, lds);
+        stages_ct<V, L, E, DIR, TWSCALE, Ns * R, Rest...>(v, t, tw, lds);
     }
 }
 
@@ -260,11 +295,11 @@ struct CtFFT {
     static constexpr int E = E_;
     static __device__ __forceinline__ int threads(const pll_fftplan &) { return L_ / E_; }
     static __device__ __forceinline__ int length(const pll_fftplan &) { return L_; }
-    template <int DIR, int TWSCALE>
-    static __device__ __forceinline__ void run(cd (&v)[E_], int t, const pll_fftplan &,
-                                               const cd *__restrict__ tw, cd *lds)
+    template <int DIR, int TWSCALE, typename V>
+    static __device__ __forceinline__ void run(V (&v)[E_], int t, const pll_fftplan &,
+                                               const V *__restrict__ tw, V *lds)
     {
-        stages_ct<L_, E_, DIR, TWSCALE, 1, Rs...>(v, t, tw, lds);
+        stages_ct<V, L_, E_, DIR, TWSCALE, 1, Rs...>(v, t, tw, lds);
     }
 };
 
@@ -273,26 +308,97 @@ struct RtFFT {
     static constexpr int E = E_;
     static __device__ __forceinline__ int threads(const pll_fftplan &P) { return P.T; }
     static __device__ __forceinline__ int length(const pll_fftplan &P) { return P.L; }
-    template <int DIR, int TWSCALE>
-    static __device__ __forceinline__ void run(cd (&v)[E_], int t, const pll_fftplan &P,
-                                               const cd *__restrict__ tw, cd *lds)
+    template <int DIR, int TWSCALE, typename V>
+    static __device__ __forceinline__ void run(V (&v)[E_], int t, const pll_fftplan &P,
+                                               const V *__restrict__ tw, V *lds)
     {
         fft_run<E_, DIR>(v, t, P, tw, TWSCALE, lds);
     }
 };
 
+// ---------------------------------------------------------------- x I/O
+// Sources of the x forward transform: real row (j,k) of nx values.
+struct SrcHalo64 {  // interior of a haloed FP64 array (the caller's p / rhs)
+    const double *p;
+    long long off1, off2;
+    template <typename V>
+    __device__ __forceinline__ V load(long long, long long j, long long k, int m, int) const
+    {
+        const d2u pr = *reinterpret_cast<const d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m);
+        return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
+    }
+};
+struct SrcDense32 {  // dense FP32 rows (the residual of a refinement sweep)
+    const float *r;
+    template <typename V>
+    __device__ __forceinline__ V load(long long row, long long, long long, int m, int nx) const
+    {
+        const float2 pr = *reinterpret_cast<const float2 *>(r + row * nx + 2 * m);
+        return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
+    }
+};
+
+// Destinations of the x backward transform; (a, b) = normalised (x_2m, x_2m+1).
+struct DstHalo64 {  // haloed FP64 array interior := value
+    double *p;
+    long long off1, off2;
+    __device__ __forceinline__ void store(long long, long long j, long long k, int m, int,
+                                          double a, double b) const
+    {
+        d2u pr;
+        pr.x = a;
+        pr.y = b;
+        *reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m) = pr;
+    }
+};
+struct DstDenseSet {  // dense FP64 := value
+    double *P;
+    __device__ __forceinline__ void store(long long row, long long, long long, int m, int nx,
+                                          double a, double b) const
+    {
+        *reinterpret_cast<double2 *>(P + row * nx + 2 * m) = make_double2(a, b);
+    }
+};
+struct DstDenseAdd {  // dense FP64 += value
+    double *P;
+    __device__ __forceinline__ void store(long long row, long long, long long, int m, int nx,
+                                          double a, double b) const
+    {
+        double2 *q = reinterpret_cast<double2 *>(P + row * nx + 2 * m);
+        const double2 o = *q;
+        *q = make_double2(o.x + a, o.y + b);
+    }
+};
+struct DstHaloFromDense {  // haloed FP64 interior := dense FP64 + value - eps
+    double *p;
+    long long off1, off2;
+    const double *P;
+    const double *eps;  // gauge constant (device scalar), may be null
+    __device__ __forceinline__ void store(long long row, long long j, long long k, int m, int nx,
+                                          double a, double b) const
+    {
+        const double2 o = *reinterpret_cast<const double2 *>(P + row * nx + 2 * m);
+        const double e = eps ? *eps : 0.0;
+        d2u pr;
+        pr.x = (o.x - e) + a;
+        pr.y = (o.y - e) + b;
+        *reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m) = pr;
+    }
+};
+
+template <typename V>
 struct XArgs {
     pll_fftplan P;
-    const cd *tw;       // [nx] exp(-2 pi i t/nx)
+    const V *tw;        // [nx] exp(-2 pi i t/nx)
     long long nrows;    // ny*nz
-    int ny, rows_per_wg, pitch;
-    long long off1, off2;  // haloed strides: (nx+2)(ny+2), nx+2
+    int nx, ny, rows_per_wg, pitch;
     double norm;
 };
 
+template <typename V>
 struct YArgs {
     pll_fftplan P;
-    const cd *tw;       // [ny] exp(-2 pi i t/ny)
+    const V *tw;        // [ny] exp(-2 pi i t/ny)
     int L, ny, cols, ntiles, pitch;
 };
 
@@ -307,112 +413,107 @@ constexpr int max_threads_f()
     return max_threads<F::E>();
 }
 
-// x forward: haloed real rows -> packed half spectrum rows of W.
-template <class F>
+// x forward: real rows -> packed half spectrum rows of W.
+template <class F, typename V, class Src>
 __global__ void __launch_bounds__(max_threads_f<F>())
-k_xfwd(const double *__restrict__ p, cd *__restrict__ W, XArgs a)
+k_xfwd(Src src, V *__restrict__ W, XArgs<V> a)
 {
-    extern __shared__ cd lds[];
+    using S = typename Sc<V>::T;
+    extern __shared__ unsigned char lds_raw[];
+    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
     const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
-    cd *ml = lds + rl * a.pitch;
+    V *ml = lds + rl * a.pitch;
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
-    const double *src = p + (k + 1) * a.off1 + (j + 1) * a.off2 + 1;
-    cd v[E];
+    V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
-        if (valid) {
-            const d2u pr = *reinterpret_cast<const d2u *>(src + 2 * m);
-            v[e] = make_double2(pr.x, pr.y);
-        } else {
-            v[e] = make_double2(0.0, 0.0);
-        }
+        v[e] = valid ? src.template load<V>(row, j, k, m, a.nx) : mk<V>(S(0), S(0));
     }
     F::template run<-1, 2>(v, t, a.P, a.tw, ml);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
     __syncthreads();
-    cd *dst = W + row * L;
+    V *dst = W + row * L;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
-        const cd Z = v[e];
-        cd X;
+        const V Z = v[e];
+        V X;
         if (m == 0) {
-            X = make_double2(Z.x + Z.y, Z.x - Z.y);  // (X_0, X_L)
+            X = mk<V>(Z.x + Z.y, Z.x - Z.y);  // (X_0, X_L)
         } else {
-            const cd Zc = cconj(ml[L - m]);
-            const cd Ev = cscale(cadd(Z, Zc), 0.5);
-            const cd D = csub(Z, Zc);
-            const cd O = make_double2(0.5 * D.y, -0.5 * D.x);  // -i D / 2
+            const V Zc = cconj(ml[L - m]);
+            const V Ev = cscale(cadd(Z, Zc), S(0.5));
+            const V D = csub(Z, Zc);
+            const V O = mk<V>(S(0.5) * D.y, S(-0.5) * D.x);  // -i D / 2
             X = cadd(Ev, cmul(a.tw[m], O));
         }
         if (valid) dst[m] = X;
     }
 }
 
-// x backward: packed half spectrum rows -> haloed real rows * norm.
-template <class F>
+// x backward: packed half spectrum rows -> real rows * norm, stored by Dst.
+template <class F, typename V, class Dst>
 __global__ void __launch_bounds__(max_threads_f<F>())
-k_xbwd(const cd *__restrict__ W, double *__restrict__ p, XArgs a)
+k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
 {
-    extern __shared__ cd lds[];
+    using S = typename Sc<V>::T;
+    extern __shared__ unsigned char lds_raw[];
+    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
     const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
-    cd *ml = lds + rl * a.pitch;
+    V *ml = lds + rl * a.pitch;
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
-    const cd *src = W + row * L;
-    cd v[E];
+    const V *src = W + row * L;
+    V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
-        v[e] = valid ? src[m] : make_double2(0.0, 0.0);
-        ml[m] = v[e];
+        ml[m] = valid ? src[m] : mk<V>(S(0), S(0));
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
-        const cd X = v[e];
+        const V X = ml[m];
         if (m == 0) {
-            v[e] = make_double2(X.x + X.y, X.x - X.y);
+            v[e] = mk<V>(X.x + X.y, X.x - X.y);
         } else {
-            const cd Xc = cconj(ml[L - m]);
-            const cd S = cadd(X, Xc);
-            const cd D = csub(X, Xc);
-            const cd Tt = cmul(cconj(a.tw[m]), D);
-            v[e] = make_double2(S.x - Tt.y, S.y + Tt.x);  // S + i Tt
+            const V Xc = cconj(ml[L - m]);
+            const V Sm = cadd(X, Xc);
+            const V D = csub(X, Xc);
+            const V Tt = cmul(cconj(a.tw[m]), D);
+            v[e] = mk<V>(Sm.x - Tt.y, Sm.y + Tt.x);  // S + i Tt
         }
     }
     __syncthreads();
     F::template run<+1, 2>(v, t, a.P, a.tw, ml);
     if (valid) {
-        double *dst = p + (k + 1) * a.off1 + (j + 1) * a.off2 + 1;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int m = t + e * T;
-            d2u pr;
-            pr.x = v[e].x * a.norm;
-            pr.y = v[e].y * a.norm;
-            *reinterpret_cast<d2u *>(dst + 2 * m) = pr;
+            dst.store(row, j, k, m, a.nx, (double)v[e].x * a.norm, (double)v[e].y * a.norm);
         }
     }
 }
 
 // y forward, in place on W: tile of `cols` adjacent columns of one k-plane.
-template <class F>
+template <class F, typename V>
 __global__ void __launch_bounds__(max_threads_f<F>())
-k_yfwd(cd *__restrict__ W, YArgs a)
+k_yfwd(V *__restrict__ W, YArgs<V> a)
 {
-    extern __shared__ cd lds[];
+    using S = typename Sc<V>::T;
+    extern __shared__ unsigned char lds_raw[];
+    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
     const int tile = blockIdx.x % a.ntiles;
@@ -420,13 +521,13 @@ k_yfwd(cd *__restrict__ W, YArgs a)
     const int c = threadIdx.x % a.cols, t = threadIdx.x / a.cols;
     const int mx = tile * a.cols + c;
     const bool valid = mx < L;
-    cd *col = W + k * (long long)ny * L + (valid ? mx : 0);
-    cd *ml = lds + c * a.pitch;
-    cd v[E];
+    V *col = W + k * (long long)ny * L + (valid ? mx : 0);
+    V *ml = lds + c * a.pitch;
+    V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int my = t + e * T;
-        v[e] = valid ? col[(long long)my * L] : make_double2(0.0, 0.0);
+        v[e] = valid ? col[(long long)my * L] : mk<V>(S(0), S(0));
     }
     F::template run<-1, 1>(v, t, a.P, a.tw, ml);
     if (tile == 0) {
@@ -442,10 +543,10 @@ k_yfwd(cd *__restrict__ W, YArgs a)
                 const int my = t + e * T;
                 if (my == 0 || 2 * my == ny) continue;  // (A, B) = (Re C, Im C)
                 if (2 * my < ny) {
-                    v[e] = cscale(cadd(v[e], cconj(ml[ny - my])), 0.5);  // A_my
+                    v[e] = cscale(cadd(v[e], cconj(ml[ny - my])), S(0.5));  // A_my
                 } else {
-                    const cd D = csub(ml[ny - my], cconj(v[e]));
-                    v[e] = make_double2(0.5 * D.y, -0.5 * D.x);  // B_{ny-my}
+                    const V D = csub(ml[ny - my], cconj(v[e]));
+                    v[e] = mk<V>(S(0.5) * D.y, S(-0.5) * D.x);  // B_{ny-my}
                 }
             }
         }
@@ -456,11 +557,13 @@ k_yfwd(cd *__restrict__ W, YArgs a)
     }
 }
 
-template <class F>
+template <class F, typename V>
 __global__ void __launch_bounds__(max_threads_f<F>())
-k_ybwd(cd *__restrict__ W, YArgs a)
+k_ybwd(V *__restrict__ W, YArgs<V> a)
 {
-    extern __shared__ cd lds[];
+    using S = typename Sc<V>::T;
+    extern __shared__ unsigned char lds_raw[];
+    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
     const int tile = blockIdx.x % a.ntiles;
@@ -468,13 +571,13 @@ k_ybwd(cd *__restrict__ W, YArgs a)
     const int c = threadIdx.x % a.cols, t = threadIdx.x / a.cols;
     const int mx = tile * a.cols + c;
     const bool valid = mx < L;
-    cd *col = W + k * (long long)ny * L + (valid ? mx : 0);
-    cd *ml = lds + c * a.pitch;
-    cd v[E];
+    V *col = W + k * (long long)ny * L + (valid ? mx : 0);
+    V *ml = lds + c * a.pitch;
+    V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int my = t + e * T;
-        v[e] = valid ? col[(long long)my * L] : make_double2(0.0, 0.0);
+        v[e] = valid ? col[(long long)my * L] : mk<V>(S(0), S(0));
     }
     if (tile == 0) {
         // rebuild C = A + i B over all my from the packed halves
@@ -487,12 +590,12 @@ k_ybwd(cd *__restrict__ W, YArgs a)
                 const int my = t + e * T;
                 if (my == 0 || 2 * my == ny) continue;
                 if (2 * my < ny) {
-                    const cd B = ml[ny - my];
-                    v[e] = make_double2(v[e].x - B.y, v[e].y + B.x);
+                    const V B = ml[ny - my];
+                    v[e] = mk<V>(v[e].x - B.y, v[e].y + B.x);
                 } else {
-                    const cd A = cconj(ml[ny - my]);
-                    const cd B = cconj(v[e]);
-                    v[e] = make_double2(A.x - B.y, A.y + B.x);
+                    const V A = cconj(ml[ny - my]);
+                    const V B = cconj(v[e]);
+                    v[e] = mk<V>(A.x - B.y, A.y + B.x);
                 }
             }
         }
@@ -505,39 +608,53 @@ k_ybwd(cd *__restrict__ W, YArgs a)
     }
 }
 
-template <class F>
-int launch_x(bool fwd, const pll_ppe_dev *d, const double *p_in, double *p_out, void *W,
-             hipStream_t s)
+template <typename V>
+XArgs<V> x_args(const pll_ppe_dev *d, const void *tw)
 {
-    XArgs a;
+    XArgs<V> a;
     a.P = d->px;
-    a.tw = (const cd *)d->tw_x;
+    a.tw = (const V *)tw;
     a.nrows = (long long)d->ny * d->nz;
+    a.nx = d->nx;
     a.ny = d->ny;
     a.rows_per_wg = d->x_rows_per_wg;
     a.pitch = d->px.L + 1;
-    a.off1 = (long long)(d->nx + 2) * (d->ny + 2);
-    a.off2 = d->nx + 2;
     a.norm = d->normfactor;
+    return a;
+}
+
+template <class F, typename V, class Src>
+int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_t s)
+{
+    const XArgs<V> a = x_args<V>(d, tw);
     const int threads = a.rows_per_wg * a.P.T;
     const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
-    const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(cd);
+    const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(V);
     if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
-    if (fwd)
-        hipLaunchKernelGGL(k_xfwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s, p_in,
-                           (cd *)W, a);
-    else
-        hipLaunchKernelGGL(k_xbwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s,
-                           (const cd *)W, p_out, a);
+    hipLaunchKernelGGL((k_xfwd<F, V, Src>), dim3((unsigned)blocks), dim3(threads), shm, s, src,
+                       (V *)W, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-template <class F>
-int launch_y(bool fwd, const pll_ppe_dev *d, void *W, hipStream_t s)
+template <class F, typename V, class Dst>
+int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipStream_t s)
 {
-    YArgs a;
+    const XArgs<V> a = x_args<V>(d, tw);
+    const int threads = a.rows_per_wg * a.P.T;
+    const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
+    const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(V);
+    if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    hipLaunchKernelGGL((k_xbwd<F, V, Dst>), dim3((unsigned)blocks), dim3(threads), shm, s,
+                       (const V *)W, dst, a);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <class F, typename V>
+int launch_y(bool fwd, const pll_ppe_dev *d, const void *tw, void *W, hipStream_t s)
+{
+    YArgs<V> a;
     a.P = d->py;
-    a.tw = (const cd *)d->tw_y;
+    a.tw = (const V *)tw;
     a.L = d->L;
     a.ny = d->ny;
     a.cols = d->y_cols_per_wg;
@@ -545,31 +662,33 @@ int launch_y(bool fwd, const pll_ppe_dev *d, void *W, hipStream_t s)
     a.pitch = d->ny + 1;
     const int threads = a.cols * a.P.T;
     const long long blocks = (long long)a.ntiles * d->nz;
-    const size_t shm = (size_t)a.cols * a.pitch * sizeof(cd);
+    const size_t shm = (size_t)a.cols * a.pitch * sizeof(V);
     if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     if (fwd)
-        hipLaunchKernelGGL(k_yfwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s, (cd *)W, a);
+        hipLaunchKernelGGL((k_yfwd<F, V>), dim3((unsigned)blocks), dim3(threads), shm, s, (V *)W, a);
     else
-        hipLaunchKernelGGL(k_ybwd<F>, dim3((unsigned)blocks), dim3(threads), shm, s, (cd *)W, a);
+        hipLaunchKernelGGL((k_ybwd<F, V>), dim3((unsigned)blocks), dim3(threads), shm, s, (V *)W, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 // Compile-time plans for the transform lengths of the production grids
-// (x: L = nx/2, y: L = ny).  Every entry must match what ct_plan() reports.
+// (x: L = nx/2, y: L = ny).  For a length listed twice the first entry is the
+// default (fewer registers, more waves per SIMD: faster on gfx950 as measured)
+// and PLL_FFT_E selects the other.
 #define PLL_CT_PLANS(X)                 \
     X(32, 8, 8, 4)                      \
     X(48, 24, 8, 6)                     \
     X(64, 8, 8, 8)                      \
     X(96, 24, 8, 4, 3)                  \
     X(128, 8, 8, 8, 2)                  \
-    X(192, 24, 8, 8, 3)                 \
     X(192, 12, 4, 4, 4, 3)              \
+    X(192, 24, 8, 8, 3)                 \
     X(256, 8, 8, 8, 4)                  \
-    X(384, 24, 8, 8, 6)                 \
     X(384, 12, 4, 4, 4, 6)              \
+    X(384, 24, 8, 8, 6)                 \
     X(512, 8, 8, 8, 8)                  \
-    X(768, 24, 8, 8, 4, 3)              \
     X(768, 12, 4, 4, 4, 4, 3)           \
+    X(768, 24, 8, 8, 4, 3)              \
     X(1024, 8, 8, 8, 8, 2)
 
 template <typename G>
@@ -612,39 +731,90 @@ extern "C" int pll_fft_ct_plan(int L, int prefer_E, pll_fftplan *p)
         if (plans[i][0] != L) continue;
         if (pick < 0 || plans[i][1] == prefer_E) pick = (int)i;
     }
-    for (size_t i = (size_t)(pick < 0 ? 0 : pick); pick >= 0 && i == (size_t)pick; i++) {
-        p->L = L;
-        p->E = plans[i][1];
-        p->T = L / p->E;
-        int n = 0, prod = 1;
-        for (int j = 2; j < 8 && plans[i][j] > 0; j++) {
-            p->rad[n++] = plans[i][j];
-            prod *= plans[i][j];
-        }
-        p->nst = n;
-        return prod == L;
+    if (pick < 0) return 0;
+    p->L = L;
+    p->E = plans[pick][1];
+    p->T = L / p->E;
+    int n = 0, prod = 1;
+    for (int j = 2; j < 8 && plans[pick][j] > 0; j++) {
+        p->rad[n++] = plans[pick][j];
+        prod *= plans[pick][j];
     }
-    return 0;
+    p->nst = n;
+    return prod == L;
 }
 
+// ---- FP64 solve
 extern "C" int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s)
 {
-    return dispatch_plan(d->px, [&](auto f) { return launch_x<decltype(f)>(true, d, p, nullptr, W, s); });
+    const SrcHalo64 src{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s);
+    });
 }
 
 extern "C" int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s)
 {
+    const DstHalo64 dst{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
     return dispatch_plan(d->px, [&](auto f) {
-        return launch_x<decltype(f)>(false, d, nullptr, p, const_cast<void *>(W), s);
+        return launch_xb<decltype(f), double2>(d, d->tw_x, W, dst, s);
     });
 }
 
 extern "C" int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
-    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f)>(true, d, W, s); });
+    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f), double2>(true, d, d->tw_y, W, s); });
 }
 
 extern "C" int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
-    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f)>(false, d, W, s); });
+    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f), double2>(false, d, d->tw_y, W, s); });
+}
+
+// ---- FP32 solve of the refined form
+extern "C" int pll_launch_xfwd32(const pll_ppe_dev *d, int src_kind, const void *src, void *W32,
+                                 hipStream_t s)
+{
+    if (src_kind == PLL_SRC_HALO64) {
+        const SrcHalo64 in{(const double *)src, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+        return dispatch_plan(d->px, [&](auto f) {
+            return launch_xf<decltype(f), float2>(d, d->tw_x32, in, W32, s);
+        });
+    }
+    const SrcDense32 in{(const float *)src};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xf<decltype(f), float2>(d, d->tw_x32, in, W32, s);
+    });
+}
+
+extern "C" int pll_launch_xbwd32(const pll_ppe_dev *d, int dst_kind, const void *W32,
+                                 double *p_halo, double *P_dense, const double *eps,
+                                 hipStream_t s)
+{
+    const long long off1 = (long long)(d->nx + 2) * (d->ny + 2), off2 = d->nx + 2;
+    return dispatch_plan(d->px, [&](auto f) {
+        using F = decltype(f);
+        switch (dst_kind) {
+            case PLL_DST_HALO_SET:
+                return launch_xb<F, float2>(d, d->tw_x32, W32, DstHalo64{p_halo, off1, off2}, s);
+            case PLL_DST_DENSE_SET:
+                return launch_xb<F, float2>(d, d->tw_x32, W32, DstDenseSet{P_dense}, s);
+            case PLL_DST_DENSE_ADD:
+                return launch_xb<F, float2>(d, d->tw_x32, W32, DstDenseAdd{P_dense}, s);
+            case PLL_DST_HALO_FROM_DENSE:
+                return launch_xb<F, float2>(d, d->tw_x32, W32,
+                                            DstHaloFromDense{p_halo, off1, off2, P_dense, eps}, s);
+            default: return -1;
+        }
+    });
+}
+
+extern "C" int pll_launch_yfwd32(const pll_ppe_dev *d, void *W32, hipStream_t s)
+{
+    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f), float2>(true, d, d->tw_y32, W32, s); });
+}
+
+extern "C" int pll_launch_ybwd32(const pll_ppe_dev *d, void *W32, hipStream_t s)
+{
+    return dispatch_plan(d->py, [&](auto f) { return launch_y<decltype(f), float2>(false, d, d->tw_y32, W32, s); });
 }

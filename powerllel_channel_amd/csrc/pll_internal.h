@@ -41,7 +41,12 @@ typedef struct {
     int y_cols_per_wg;         /* columns per workgroup, y kernels  */
     const void *tw_x;          /* double2[nx]: exp(-2 pi i t / nx)  */
     const void *tw_y;          /* double2[ny]: exp(-2 pi i t / ny)  */
+    const void *tw_x32;        /* float2[nx], same table in FP32 (refined form) */
+    const void *tw_y32;        /* float2[ny] */
+    double idx2, idy2;         /* 1/dx^2, 1/dy^2 (residual of the refined form) */
     const double *a, *c;       /* [nz] tridiagonal off-diagonals    */
+    const double *bz;          /* [nz] diagonal before adding the eigenvalue */
+    const double *dz;          /* [nz] row sums a + bz + c (zero up to rounding) */
     const double *piv;         /* [nz][ny/2+1][L+1] factorised diagonal */
     struct {                   /* partitioned z solve (P = 0: streaming solve) */
         int M, P, S;           /* rows per lane, lanes per column, columns per tile */
@@ -54,6 +59,30 @@ typedef struct {
 } pll_ppe_dev;
 
 int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s);
+
+/* FP32 solve of the refined form.  W32 is the spectral array in FP32 (N
+ * floats, same layout as W).  x forward sources and x backward destinations: */
+enum { PLL_SRC_HALO64 = 0, PLL_SRC_DENSE32 = 1 };
+enum {
+    PLL_DST_HALO_SET = 0,        /* haloed p interior := c               */
+    PLL_DST_DENSE_SET = 1,       /* dense P := c                          */
+    PLL_DST_DENSE_ADD = 2,       /* dense P += c                          */
+    PLL_DST_HALO_FROM_DENSE = 3  /* haloed p interior := dense P + c      */
+};
+int pll_launch_xfwd32(const pll_ppe_dev *d, int src_kind, const void *src, void *W32,
+                      hipStream_t s);
+int pll_launch_yfwd32(const pll_ppe_dev *d, void *W32, hipStream_t s);
+int pll_launch_trid32(const pll_ppe_dev *d, void *W32, hipStream_t s);
+int pll_launch_ybwd32(const pll_ppe_dev *d, void *W32, hipStream_t s);
+/* eps: device scalar subtracted by PLL_DST_HALO_FROM_DENSE (NULL: none). */
+int pll_launch_xbwd32(const pll_ppe_dev *d, int dst_kind, const void *W32, double *p_halo,
+                      double *P_dense, const double *eps, hipStream_t s);
+/* r = f - L_h P in FP64, stored as FP32 (dense, x fastest); f is the interior
+ * of the haloed array, P dense.  gauge non-NULL (pll_gauge_parts(d) + 1
+ * doubles): also gauge[parts] := mean of P over the plane k = nz-1. */
+int pll_launch_resid(const pll_ppe_dev *d, const double *f_halo, const double *P, float *r,
+                     double *gauge, hipStream_t s);
+int pll_gauge_parts(const pll_ppe_dev *d);
 int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s);

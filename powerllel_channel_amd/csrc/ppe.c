@@ -130,6 +130,17 @@ static int trid_coeff(int n, const double *dzf, const char *bct, double *a, doub
     return PLL_OK;
 }
 
+/* a + b + c to within one rounding of the exact sum (Knuth's TwoSum on both
+ * additions).  The row sums of the z operator are zero up to the rounding of
+ * b = -a - c, and that rounding is part of the operator the solve inverts, so
+ * the residual of the refined form needs it exactly. */
+static double rowsum3(double a, double b, double c)
+{
+    const double s = a + b, bv = s - a, e1 = (a - (s - bv)) + (b - bv);
+    const double t = s + c, cv = t - s, e2 = (s - (t - cv)) + (c - cv);
+    return t + (e1 + e2);
+}
+
 /* ------------------------------------------------------------------ handle */
 #define PLL_TIMING_RING 128
 struct pll_ppe {
@@ -137,6 +148,9 @@ struct pll_ppe {
     pll_ppe_dev d;
     void *W, *tw_x, *tw_y, *a, *c, *piv;
     void *bz, *lamx, *lamy, *order;   /* partitioned z solve */
+    void *tw_x32, *tw_y32, *W32, *P64, *r32; /* refined form (allocated on first use) */
+    void *gauge;                      /* top-plane partial sums + mean (refined form) */
+    int sweeps;                       /* pll_ppe_solve: -1 FP64, >= 0 FP32 + sweeps */
     double *stage;   /* haloed staging buffer for host-pointer calls */
     size_t stage_bytes;
     size_t dev_bytes;
@@ -160,7 +174,8 @@ int pll_ppe_destroy(pll_ppe *h)
 {
     if (!h) return PLL_OK;
     void *bufs[] = {h->W, h->tw_x, h->tw_y, h->a, h->c, h->piv, h->stage,
-                    h->bz, h->lamx, h->lamy, h->order};
+                    h->bz, h->lamx, h->lamy, h->order, h->tw_x32, h->tw_y32,
+                    h->W32, h->P64, h->r32, h->gauge};
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
         if (bufs[i]) hipFree(bufs[i]);
     if (h->timing)
@@ -195,7 +210,8 @@ static int choose_launch(pll_ppe *h)
 /* Layout of the partitioned z solve: M rows per lane (M in {8,12,16,4}),
  * P = nz/M lanes per column (a power of two, 2..64, so a column's lanes share
  * one wavefront), S columns per tile (16, 8 or 4 doubles = one 128/64/32-byte
- * row segment).  Returns 0 (streaming solve) if nothing fits. */
+ * row segment); LDS holds the tile twice (values and, for low-mode tiles,
+ * pivots).  Returns 0 (streaming solve) if nothing fits. */
 static int choose_part(int nx, int nz, int *M, int *P, int *S)
 {
     static const int Ms[] = {8, 12, 16, 4};
@@ -207,7 +223,7 @@ static int choose_part(int nx, int nz, int *M, int *P, int *S)
         if (p < 2 || p > 64 || (p & (p - 1))) continue;
         for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
             const int sc = Ss[j];
-            if (nx % sc || sc * p > 512 || (size_t)nz * sc * 8 > 65536) continue;
+            if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 > 163840) continue;
             *M = m;
             *P = p;
             *S = sc;
@@ -266,7 +282,7 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     const size_t npiv = (size_t)nz * nyh * (L + 1);
     double *a = (double *)malloc(sizeof(double) * (size_t)nz);
     double *c = (double *)malloc(sizeof(double) * (size_t)nz);
-    double *bz = (double *)malloc(sizeof(double) * (size_t)nz);
+    double *bz = (double *)malloc(sizeof(double) * 2 * (size_t)nz); /* bz, then row sums */
     double *lx = (double *)malloc(sizeof(double) * (size_t)nx);
     double *ly = (double *)malloc(sizeof(double) * (size_t)ny);
     double *piv = (double *)malloc(sizeof(double) * npiv);
@@ -276,6 +292,7 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
         goto fail;
     }
     if ((rc = trid_coeff(nz, dzf_global, bctype_z, a, bz, c))) goto fail;
+    for (int k = 0; k < nz; k++) bz[nz + k] = rowsum3(a[k], bz[k], c[k]);
     eigen_pp(nx, dx, lx);
     eigen_pp(ny, dy, ly);
     /* pivots: b_k = bz_k + lambda; b_k -= (a_k / b_{k-1}) c_{k-1}
@@ -295,7 +312,11 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
             }
         }
     }
-    if ((rc = dmalloc(h, &h->W, sizeof(double) * N)) ||
+    h->sweeps = -1;
+    h->d.idx2 = 1.0 / (dx * dx);
+    h->d.idy2 = 1.0 / (dy * dy);
+    if ((rc = dmalloc(h, &h->bz, sizeof(double) * 2 * (size_t)nz)) ||
+        (rc = dmalloc(h, &h->W, sizeof(double) * N)) ||
         (rc = dmalloc(h, &h->tw_x, sizeof(double) * 2 * (size_t)nx)) ||
         (rc = dmalloc(h, &h->tw_y, sizeof(double) * 2 * (size_t)ny)) ||
         (rc = dmalloc(h, &h->a, sizeof(double) * (size_t)nz)) ||
@@ -315,12 +336,15 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     H2D(h->a, a, sizeof(double) * (size_t)nz);
     H2D(h->c, c, sizeof(double) * (size_t)nz);
     H2D(h->piv, piv, sizeof(double) * npiv);
+    H2D(h->bz, bz, sizeof(double) * 2 * (size_t)nz);
 #undef H2D
     h->d.tw_x = h->tw_x;
     h->d.tw_y = h->tw_y;
     h->d.a = (const double *)h->a;
     h->d.c = (const double *)h->c;
     h->d.piv = (const double *)h->piv;
+    h->d.bz = (const double *)h->bz;
+    h->d.dz = (const double *)h->bz + nz;
     {
         int M, P, S;
         if (!getenv("PLL_TRID_STREAMING") && choose_part(nx, nz, &M, &P, &S)) {
@@ -355,11 +379,9 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
                 goto fail;
             }
             for (int i = 0; i <= L; i++) lxh[i] = lx[i];
-            if ((rc = dmalloc(h, &h->bz, sizeof(double) * (size_t)nz)) ||
-                (rc = dmalloc(h, &h->lamx, sizeof(double) * (size_t)(L + 1))) ||
+            if ((rc = dmalloc(h, &h->lamx, sizeof(double) * (size_t)(L + 1))) ||
                 (rc = dmalloc(h, &h->lamy, sizeof(double) * (size_t)nyh)) ||
                 (rc = dmalloc(h, &h->order, sizeof(int) * (size_t)ntiles)) ||
-                (hipMemcpy(h->bz, bz, sizeof(double) * (size_t)nz, hipMemcpyHostToDevice) != hipSuccess) ||
                 (hipMemcpy(h->lamx, lxh, sizeof(double) * (size_t)(L + 1), hipMemcpyHostToDevice) != hipSuccess) ||
                 (hipMemcpy(h->lamy, ly, sizeof(double) * (size_t)nyh, hipMemcpyHostToDevice) != hipSuccess) ||
                 (hipMemcpy(h->order, order, sizeof(int) * (size_t)ntiles, hipMemcpyHostToDevice) != hipSuccess)) {
@@ -447,10 +469,13 @@ static int launch_rc(int rc, const char *what)
                          hipGetErrorString(hipGetLastError()));
 }
 
+int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream);
+
 int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
 {
     if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
     if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
+    if (h->sweeps >= 0) return pll_ppe_solve_refined(h, d_var, h->sweeps, stream);
     hipStream_t s = (hipStream_t)stream;
     int rc;
     hipEvent_t *ev = h->timing ? h->ev[h->nrec % PLL_TIMING_RING] : NULL;
@@ -469,6 +494,108 @@ int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
         h->nrec++;
     }
     return PLL_OK;
+}
+
+/* ------------------------------------------------------------------ refined form */
+/* FP32 solve + FP64 residual sweeps.  The spectral array, the FFT arithmetic
+ * and the correction solves are FP32 (the z recurrence still runs in FP64
+ * registers); f stays the caller's FP64 right-hand side, p is accumulated in
+ * an FP64 work array and every sweep solves L c = f - L p for a correction. */
+static int ensure_fp32(pll_ppe *h)
+{
+    if (h->W32) return PLL_OK;
+    const size_t N = (size_t)h->nx * h->ny * h->nz;
+    int rc;
+    float *tx = (float *)malloc(sizeof(float) * 2 * (size_t)h->nx);
+    float *ty = (float *)malloc(sizeof(float) * 2 * (size_t)h->ny);
+    double *dx64 = twiddles(h->nx), *dy64 = twiddles(h->ny);
+    if (!tx || !ty || !dx64 || !dy64) {
+        free(tx); free(ty); free(dx64); free(dy64);
+        return pll_set_error(PLL_ENOMEM, "host allocation failed");
+    }
+    for (int i = 0; i < 2 * h->nx; i++) tx[i] = (float)dx64[i];
+    for (int i = 0; i < 2 * h->ny; i++) ty[i] = (float)dy64[i];
+    free(dx64);
+    free(dy64);
+    /* a failed earlier attempt may have left some of these allocated */
+    if ((!h->tw_x32 && (rc = dmalloc(h, &h->tw_x32, sizeof(float) * 2 * (size_t)h->nx))) ||
+        (!h->tw_y32 && (rc = dmalloc(h, &h->tw_y32, sizeof(float) * 2 * (size_t)h->ny))) ||
+        (!h->P64 && (rc = dmalloc(h, &h->P64, sizeof(double) * N))) ||
+        (!h->r32 && (rc = dmalloc(h, &h->r32, sizeof(float) * N))) ||
+        (!h->gauge && (rc = dmalloc(h, &h->gauge, sizeof(double) * (size_t)(pll_gauge_parts(&h->d) + 1)))) ||
+        (rc = dmalloc(h, &h->W32, sizeof(float) * N))) {
+        free(tx); free(ty);
+        return rc;
+    }
+    hipError_t e1 = hipMemcpy(h->tw_x32, tx, sizeof(float) * 2 * (size_t)h->nx, hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(h->tw_y32, ty, sizeof(float) * 2 * (size_t)h->ny, hipMemcpyHostToDevice);
+    free(tx);
+    free(ty);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        hipFree(h->W32);  /* W32 marks the set complete */
+        h->W32 = NULL;
+        return pll_set_error(PLL_EHIP, "copy of FP32 twiddles failed");
+    }
+    h->d.tw_x32 = h->tw_x32;
+    h->d.tw_y32 = h->tw_y32;
+    return PLL_OK;
+}
+
+static int spectral32(pll_ppe *h, hipStream_t s)
+{
+    int rc;
+    if ((rc = launch_rc(pll_launch_yfwd32(&h->d, h->W32, s), "y forward (fp32)"))) return rc;
+    if ((rc = launch_rc(pll_launch_trid32(&h->d, h->W32, s), "z solve (fp32 storage)"))) return rc;
+    return launch_rc(pll_launch_ybwd32(&h->d, h->W32, s), "y backward (fp32)");
+}
+
+int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream)
+{
+    if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
+    if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
+    if (sweeps < 0 || sweeps > 64) return pll_set_error(PLL_EINVAL, "sweeps must be in [0, 64]");
+    int rc;
+    if ((rc = ensure_fp32(h))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_HALO64, d_var, h->W32, s), "x forward (fp32)")))
+        return rc;
+    if ((rc = spectral32(h, s))) return rc;
+    if (sweeps == 0)
+        return launch_rc(pll_launch_xbwd32(&h->d, PLL_DST_HALO_SET, h->W32, d_var, NULL, NULL, s),
+                         "x backward (fp32)");
+    if ((rc = launch_rc(pll_launch_xbwd32(&h->d, PLL_DST_DENSE_SET, h->W32, NULL, (double *)h->P64,
+                                          NULL, s),
+                        "x backward (fp32)")))
+        return rc;
+    /* Gauge: corrections keep the constant of the (0,0) mode that the first
+     * solve left, i.e. the FP32 roundoff of its top-plane mean; the last
+     * sweep measures the top-plane mean of P and the final write removes it. */
+    double *gauge = h->d.pin_gauge ? (double *)h->gauge : NULL;
+    const double *eps = gauge ? gauge + pll_gauge_parts(&h->d) : NULL;
+    for (int it = 1; it <= sweeps; it++) {
+        const int last = it == sweeps;
+        if ((rc = launch_rc(pll_launch_resid(&h->d, d_var, (const double *)h->P64, (float *)h->r32,
+                                             last ? gauge : NULL, s),
+                            "residual")))
+            return rc;
+        if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_DENSE32, h->r32, h->W32, s),
+                            "x forward (fp32)")))
+            return rc;
+        if ((rc = spectral32(h, s))) return rc;
+        if ((rc = launch_rc(pll_launch_xbwd32(&h->d, last ? PLL_DST_HALO_FROM_DENSE : PLL_DST_DENSE_ADD,
+                                              h->W32, d_var, (double *)h->P64, last ? eps : NULL, s),
+                            "x backward (fp32)")))
+            return rc;
+    }
+    return PLL_OK;
+}
+
+int pll_ppe_set_refinement(pll_ppe *h, int sweeps)
+{
+    if (!h) return pll_set_error(PLL_EINVAL, "NULL handle");
+    if (sweeps < -1 || sweeps > 64) return pll_set_error(PLL_EINVAL, "sweeps must be in [-1, 64]");
+    h->sweeps = sweeps;
+    return sweeps >= 0 ? ensure_fp32(h) : PLL_OK;
 }
 
 /* ------------------------------------------------------------------ drop-in */
@@ -526,6 +653,12 @@ int execute_poisson_solver_status(double *var)
     if ((rc = pll_ppe_solve(h, h->stage, NULL))) return rc;
     HIPCHK(hipMemcpy(var, h->stage, bytes, hipMemcpyDeviceToHost));
     return PLL_OK;
+}
+
+int set_poisson_solver_refinement(int sweeps)
+{
+    if (!g_ppe) return pll_set_error(PLL_ESTATE, "set_poisson_solver_refinement before init");
+    return pll_ppe_set_refinement(g_ppe, sweeps);
 }
 
 int free_poisson_solver_status(void)

@@ -32,7 +32,8 @@ struct TArgs {
     int L, ny, nz, pin_gauge;
 };
 
-__global__ void __launch_bounds__(256) k_trid(cd *__restrict__ W, TArgs g)
+template <typename C>
+__global__ void __launch_bounds__(256) k_trid(C *__restrict__ W, TArgs g)
 {
     const long long nslot = (long long)g.L * g.ny;
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,16 +58,17 @@ __global__ void __launch_bounds__(256) k_trid(cd *__restrict__ W, TArgs g)
     const bool same = (ixr == ixi);
     const double *__restrict__ a = g.a;
     const double *__restrict__ c = g.c;
-    cd *__restrict__ col = W + s;
+    C *__restrict__ col = W + s;
 
-    cd prev = col[0];
+    cd prev = make_double2(col[0].x, col[0].y);
     for (int k = 1; k < nz; ++k) {
-        cd cur = col[(long long)k * nslot];
+        cd cur = make_double2(col[(long long)k * nslot].x, col[(long long)k * nslot].y);
         const double mr = a[k] / pr[(long long)(k - 1) * pst];
         const double mi = same ? mr : a[k] / pi[(long long)(k - 1) * pst];
         cur.x -= mr * prev.x;
         cur.y -= mi * prev.y;
-        col[(long long)k * nslot] = cur;
+        col[(long long)k * nslot].x = cur.x;
+        col[(long long)k * nslot].y = cur.y;
         prev = cur;
     }
     {
@@ -75,15 +77,17 @@ __global__ void __launch_bounds__(256) k_trid(cd *__restrict__ W, TArgs g)
         const bool pin = g.pin_gauge && s == 0;
         prev.x = pin ? 0.0 : (br != 0.0 ? prev.x / br : 0.0);
         prev.y = bi != 0.0 ? prev.y / bi : 0.0;
-        col[(long long)(nz - 1) * nslot] = prev;
+        col[(long long)(nz - 1) * nslot].x = prev.x;
+        col[(long long)(nz - 1) * nslot].y = prev.y;
     }
     for (int k = nz - 2; k >= 0; --k) {
-        cd cur = col[(long long)k * nslot];
+        cd cur = make_double2(col[(long long)k * nslot].x, col[(long long)k * nslot].y);
         const double br = pr[(long long)k * pst];
         const double bi = same ? br : pi[(long long)k * pst];
         cur.x = (cur.x - c[k] * prev.x) / br;
         cur.y = (cur.y - c[k] * prev.y) / bi;
-        col[(long long)k * nslot] = cur;
+        col[(long long)k * nslot].x = cur.x;
+        col[(long long)k * nslot].y = cur.y;
         prev = cur;
     }
 }
@@ -116,6 +120,17 @@ struct PArgs {
     int qx, iyx;  // exact path for tiles with q0 < qx and iy < iyx
 };
 
+// 1/x from the hardware reciprocal plus two Newton steps (full double
+// precision, not necessarily correctly rounded); x is a nonzero pivot.
+__device__ __forceinline__ double rcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __device__ __forceinline__ int swz(int k, int s, int S, int M)
 {
     return k * S + (s ^ ((k / M) & (S - 1)));
@@ -137,8 +152,8 @@ __device__ __forceinline__ int lam_ix(int q, int my, int L, int ny)
     return (2 * my < ny) ? 0 : L;
 }
 
-template <int M>
-__global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArgs g)
+template <int M, typename St>
+__global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 {
     extern __shared__ double tl[];
     const int S = g.S, P = g.P, nz = g.nz;
@@ -146,39 +161,49 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
     const int tile = g.order[blockIdx.x];
     const int my = tile / g.ntx, q0 = (tile % g.ntx) * S;
     const long long plane = (long long)g.nx * g.ny;
-    double *__restrict__ base = Wr + (long long)my * g.nx + q0;
+    St *__restrict__ base = Wr + (long long)my * g.nx + q0;
     const int nthr = S * P;
 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         const int e = tid + j * nthr;
         const int k = e / S, s = e - k * S;
-        tl[swz(k, s, S, M)] = base[k * plane + s];
+        tl[swz(k, s, S, M)] = (double)base[k * plane + s];
     }
     __syncthreads();
 
     const int iy = my <= g.ny - my ? my : g.ny - my;
     if (q0 < g.qx && iy < g.iyx) {
+        // stored pivots of the tile's columns into the second half of LDS
+        double *pt = tl + nz * S;
+        const long long pst = (long long)(g.L + 1) * (g.ny / 2 + 1);
+        const double *__restrict__ prow = g.piv + (long long)iy * (g.L + 1);
+#pragma unroll 1
+        for (int j = 0; j < M; ++j) {
+            const int e = tid + j * nthr;
+            const int k = e / S, s = e - k * S;
+            pt[swz(k, s, S, M)] = prow[k * pst + lam_ix(q0 + s, my, g.L, g.ny)];
+        }
+        __syncthreads();
         if (tid < S) {
             const int s = tid, q = q0 + s;
-            const int ix = lam_ix(q, my, g.L, g.ny);
-            const long long pst = (long long)(g.L + 1) * (g.ny / 2 + 1);
-            const double *__restrict__ pv = g.piv + (long long)iy * (g.L + 1) + ix;
             double prev = tl[swz(0, s, S, M)];
+#pragma unroll 4
             for (int k = 1; k < nz; ++k) {
                 const int o = swz(k, s, S, M);
-                double cur = tl[o];
-                cur -= (g.a[k] / pv[(k - 1) * pst]) * prev;
+                const double m = g.a[k] / pt[swz(k - 1, s, S, M)];
+                const double cur = tl[o] - m * prev;
                 tl[o] = cur;
                 prev = cur;
             }
-            const double bl = pv[(nz - 1) * pst];
+            const double bl = pt[swz(nz - 1, s, S, M)];
             const bool pin = g.pin_gauge && my == 0 && q == 0;
             prev = pin ? 0.0 : (bl != 0.0 ? prev / bl : 0.0);
             tl[swz(nz - 1, s, S, M)] = prev;
+#pragma unroll 4
             for (int k = nz - 2; k >= 0; --k) {
                 const int o = swz(k, s, S, M);
-                const double cur = (tl[o] - g.c[k] * prev) / pv[k * pst];
+                const double cur = (tl[o] - g.c[k] * prev) / pt[o];
                 tl[o] = cur;
                 prev = cur;
             }
@@ -194,7 +219,7 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
         // 1. local elimination
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const double r = 1.0 / (g.bz[k0 + i] + lam);
+            const double r = rcp(g.bz[k0 + i] + lam);
             d[i] *= r;
             ap[i] = g.a[k0 + i] * r;
             cp[i] = g.c[k0 + i] * r;
@@ -202,7 +227,7 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
 #pragma unroll
         for (int i = 2; i < M; ++i) {
             const double ai = g.a[k0 + i];
-            const double r = 1.0 / ((g.bz[k0 + i] + lam) - ai * cp[i - 1]);
+            const double r = rcp((g.bz[k0 + i] + lam) - ai * cp[i - 1]);
             d[i] = r * (d[i] - ai * d[i - 1]);
             ap[i] = -r * ai * ap[i - 1];
             cp[i] = r * g.c[k0 + i];
@@ -214,7 +239,7 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
             cp[i] = -cp[i] * cp[i + 1];
         }
         {
-            const double r = 1.0 / (1.0 - cp[0] * ap[1]);
+            const double r = rcp(1.0 - cp[0] * ap[1]);
             d[0] = r * (d[0] - cp[0] * d[1]);
             ap[0] = r * ap[0];
             cp[0] = -r * cp[0] * cp[1];
@@ -238,11 +263,11 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
             const double pla = hasl ? ula : 0.0, plg = hasl ? ulg : 0.0, pld = hasl ? uld : 0.0;
             const double nfa = hasr ? dfa : 0.0, nfg = hasr ? dfg : 0.0, nfd = hasr ? dfd : 0.0;
             // F: left = L_{p-1}, right = L_p
-            const double rf = 1.0 / (1.0 - fa * plg - fg * la);
+            const double rf = rcp(1.0 - fa * plg - fg * la);
             const double fa2 = -fa * pla * rf, fg2 = -fg * lg * rf;
             const double fd2 = (fd - fa * pld - fg * ld) * rf;
             // L: left = F_p, right = F_{p+1}
-            const double rl = 1.0 / (1.0 - la * fg - lg * nfa);
+            const double rl = rcp(1.0 - la * fg - lg * nfa);
             const double la2 = -la * fa * rl, lg2 = -lg * nfg * rl;
             const double ld2 = (ld - la * fd - lg * nfd) * rl;
             fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
@@ -259,10 +284,10 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
             const double nfa = hasr ? dfa : 0.0, nfg = hasr ? dfg : 0.0, nfd = hasr ? dfd : 0.0;
             const double pla = hasl ? ula : 0.0, plg = hasl ? ulg : 0.0, pld = hasl ? uld : 0.0;
             const double nla = hasr ? dla : 0.0, nlg = hasr ? dlg : 0.0, nld = hasr ? dld : 0.0;
-            const double rf = 1.0 / (1.0 - fa * pfg - fg * nfa);
+            const double rf = rcp(1.0 - fa * pfg - fg * nfa);
             const double fa2 = -fa * pfa * rf, fg2 = -fg * nfg * rf;
             const double fd2 = (fd - fa * pfd - fg * nfd) * rf;
-            const double rl = 1.0 / (1.0 - la * plg - lg * nla);
+            const double rl = rcp(1.0 - la * plg - lg * nla);
             const double la2 = -la * pla * rl, lg2 = -lg * nlg * rl;
             const double ld2 = (ld - la * pld - lg * nld) * rl;
             fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
@@ -281,15 +306,12 @@ __global__ void __launch_bounds__(512) k_trid_part(double *__restrict__ Wr, PArg
     for (int j = 0; j < M; ++j) {
         const int e = tid + j * nthr;
         const int k = e / S, s = e - k * S;
-        base[k * plane + s] = tl[swz(k, s, S, M)];
+        base[k * plane + s] = (St)tl[swz(k, s, S, M)];
     }
 }
 
-}  // namespace
-
-extern "C" int pll_trid_part_supported(int M) { return M == 4 || M == 8 || M == 12 || M == 16; }
-
-extern "C" int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
+template <typename St, typename C>
+int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
     if (d->tp.P > 0) {
         PArgs g;
@@ -312,13 +334,22 @@ extern "C" int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         g.iyx = d->tp.iyx;
         const unsigned blocks = (unsigned)(g.ntx * d->ny);
         const int threads = g.S * g.P;
-        const size_t shm = sizeof(double) * (size_t)d->nz * g.S;
-        if (threads > 512 || shm > 65536) return -1;
+        const size_t shm = 2 * sizeof(double) * (size_t)d->nz * g.S;  // values + pivots
+        if (threads > 512 || shm > 163840) return -1;
         switch (d->tp.M) {
-            case 4: hipLaunchKernelGGL(k_trid_part<4>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
-            case 8: hipLaunchKernelGGL(k_trid_part<8>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
-            case 12: hipLaunchKernelGGL(k_trid_part<12>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
-            case 16: hipLaunchKernelGGL(k_trid_part<16>, dim3(blocks), dim3(threads), shm, s, (double *)W, g); break;
+#define PLL_TRID_CASE(M_)                                                                       \
+    case M_:                                                                                    \
+        if (shm > 65536 && hipFuncSetAttribute((const void *)k_trid_part<M_, St>,               \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                               (int)shm) != hipSuccess)                         \
+            return -3;                                                                          \
+        hipLaunchKernelGGL((k_trid_part<M_, St>), dim3(blocks), dim3(threads), shm, s, (St *)W, g); \
+        break;
+            PLL_TRID_CASE(4)
+            PLL_TRID_CASE(8)
+            PLL_TRID_CASE(12)
+            PLL_TRID_CASE(16)
+#undef PLL_TRID_CASE
             default: return -5;
         }
         return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -334,6 +365,20 @@ extern "C" int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
     const long long nslot = (long long)d->L * d->ny;
     const long long blocks = (nslot + 255) / 256;
     if (blocks > 0x7fffffffLL) return -1;
-    hipLaunchKernelGGL(k_trid, dim3((unsigned)blocks), dim3(256), 0, s, (cd *)W, g);
+    hipLaunchKernelGGL((k_trid<C>), dim3((unsigned)blocks), dim3(256), 0, s, (C *)W, g);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace
+
+extern "C" int pll_trid_part_supported(int M) { return M == 4 || M == 8 || M == 12 || M == 16; }
+
+extern "C" int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
+{
+    return launch_trid<double, double2>(d, W, s);
+}
+
+extern "C" int pll_launch_trid32(const pll_ppe_dev *d, void *W32, hipStream_t s)
+{
+    return launch_trid<float, float2>(d, W32, s);
 }

@@ -83,6 +83,22 @@ class PoissonSolver:
             raise ValueError("PoissonSolver.solve takes a device tensor")
         check(lib().pll_ppe_solve(self._h, addr, _stream_ptr(stream)))
 
+    def solve_refined(self, p, sweeps: int, stream=None):
+        """Refined form: FP32 transforms and solves plus `sweeps` FP64-residual
+        correction sweeps (same operator, gauge and in-place semantics)."""
+        if self._h is None:
+            raise RuntimeError("solver was closed")
+        if tuple(p.shape) != self.shape:
+            raise ValueError(f"p has shape {tuple(p.shape)}, expected {self.shape}")
+        addr, dev = _ptr_and_kind(p)
+        if not dev:
+            raise ValueError("PoissonSolver.solve_refined takes a device tensor")
+        check(lib().pll_ppe_solve_refined(self._h, addr, int(sweeps), _stream_ptr(stream)))
+
+    def set_refinement(self, sweeps: int):
+        """Make solve() use the refined form with `sweeps` sweeps (-1: FP64)."""
+        check(lib().pll_ppe_set_refinement(self._h, int(sweeps)))
+
     def set_timing(self, on: bool = True):
         check(lib().pll_ppe_set_timing(self._h, 1 if on else 0))
 
@@ -140,6 +156,12 @@ def execute_poisson_solver(var):
     a host numpy array or a device torch tensor; synchronous."""
     addr, _dev = _ptr_and_kind(var)
     check(lib().execute_poisson_solver_status(addr))
+
+
+def set_poisson_solver_refinement(sweeps: int):
+    """Select the refined form (FP32 + `sweeps` FP64-residual sweeps) for the
+    drop-in instance; -1 restores the FP64 solve."""
+    check(lib().set_poisson_solver_refinement(int(sweeps)))
 
 
 def free_poisson_solver():
