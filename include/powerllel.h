@@ -161,10 +161,80 @@ int pll_channel_max_div(pll_channel *c, const double *u, const double *v,
                         const double *w, double *value, int ijk[3],
                         void *stream);
 /* One whole time step, src/main.f90:137-208: RK1, BC, RK2, BC, calcRHS,
- * PPE (ppe must match the grid), updateBoundP, correctVel, forceVel, BC. */
+ * PPE (ppe must match the grid), updateBoundP, correctVel, forceVel, BC.
+ * With the FP64 solve, calcRHS is formed inside the solve's x transform
+ * (same arithmetic).  Forcing of u and/or v takes Mean_h from the RK2 output
+ * and shifts inside correctVel: the x (y) pressure-gradient correction sums
+ * to zero over a periodic row, so this is the same mean up to summation
+ * roundoff.  A forced w, or the refined solve, uses the separate passes. */
 int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v,
                      double *w, double *u1, double *v1, double *w1, double *p,
                      void *stream);
+
+/* ------------------------------------------------------------------------
+ * Adjoint of the projection step (no counterpart in the reference).
+ * Every array is a haloed device field like u.
+ * ---------------------------------------------------------------------- */
+/* pll_channel_step with a body force (bu, bv, bw) added to the momentum
+ * right-hand side of both RK stages (interior values are read). */
+int pll_channel_step_body(pll_channel *c, pll_ppe *ppe, double *u, double *v,
+                          double *w, double *u1, double *v1, double *w1,
+                          double *p, const double *bu, const double *bv,
+                          const double *bw, void *stream);
+/* Wall shear of src/mod_monitor.f90:225-258 (calcAndWriteSkf), averaged over
+ * the two walls: re_inv (mean_ij u(i,j,kw) + u_crf) / (dzf(kw)/2).
+ * Synchronises. */
+int pll_channel_wall_shear(pll_channel *c, const double *u, double *tau,
+                           void *stream);
+/* Transposes of the step's linear(ised) pieces (each exact for the map the
+ * forward kernel applies):
+ *  bc_vel_adj / bc_p_adj: ghost adjoints added to the interior values the
+ *    fills copy or mirror, ghosts cleared (in place);
+ *  force_vel_adj: l -= dzf/lz * sum(l) / (nx ny) for forced components;
+ *  correct_vel_adj: lp := (pressure-gradient term)^T (lu, lv, lw), every cell;
+ *  calc_rhs_adj: (lu, lv, lw) += calcRHS^T lr (lr read on the interior);
+ *  rk_jvp: o = J(q) dq on the interior, J the Jacobian of the momentum
+ *    right-hand side of timeIntVelRK1/2;
+ *  rk_vjp: o = beta o + a0 l (interior) + alpha J(q)^T l on every cell. */
+int pll_channel_bc_vel_adj(pll_channel *c, double *lu, double *lv, double *lw,
+                           void *stream);
+int pll_channel_bc_p_adj(pll_channel *c, double *lp, void *stream);
+int pll_channel_force_vel_adj(pll_channel *c, double *lu, double *lv,
+                              double *lw, void *stream);
+int pll_channel_correct_vel_adj(pll_channel *c, const double *lu,
+                                const double *lv, const double *lw, double *lp,
+                                void *stream);
+int pll_channel_calc_rhs_adj(pll_channel *c, const double *lr, double *lu,
+                             double *lv, double *lw, void *stream);
+int pll_channel_rk_jvp(pll_channel *c, const double *qu, const double *qv,
+                       const double *qw, const double *du, const double *dv,
+                       const double *dw, double *ou, double *ov, double *ow,
+                       void *stream);
+int pll_channel_rk_vjp(pll_channel *c, const double *qu, const double *qv,
+                       const double *qw, const double *lu, const double *lv,
+                       const double *lw, double *ou, double *ov, double *ow,
+                       double alpha, double a0, double beta, void *stream);
+/* Transposed PPE solve (in place, interior): the same transforms with every
+ * z system transposed; the gauge-pinned (0,0) system transposes to the same
+ * pinned form. */
+int pll_ppe_solve_adjoint(pll_ppe *h, double *d_var, void *stream);
+
+/* Gradient of J = (1/nsteps) sum_{n=1..nsteps} wall_shear(u^n) with respect
+ * to the body force, over a window of nsteps steps from (u0, v0, w0) (boundary
+ * fills applied), with checkpoints every ckpt_every steps and the RK1 stage
+ * recomputed in the reverse sweep.  ppe must use the FP64 solve.
+ * forward: runs the window, copies the body force, returns J (synchronises).
+ * backward: writes dJ/d(bu, bv, bw) (interior; halos 0). */
+typedef struct pll_adjoint pll_adjoint;
+int pll_adjoint_create(pll_adjoint **out, pll_channel *c, pll_ppe *ppe,
+                       int nsteps, int ckpt_every);
+int pll_adjoint_forward(pll_adjoint *a, const double *u0, const double *v0,
+                        const double *w0, const double *bu, const double *bv,
+                        const double *bw, double *J, void *stream);
+int pll_adjoint_backward(pll_adjoint *a, double *gu, double *gv, double *gw,
+                         void *stream);
+size_t pll_adjoint_device_bytes(const pll_adjoint *a);
+int pll_adjoint_destroy(pll_adjoint *a);
 
 #ifdef __cplusplus
 }

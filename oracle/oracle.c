@@ -709,8 +709,11 @@ static inline void mom_rhs(const orc_grid_t *g, const double *qu, const double *
 #undef V
 #undef W
 
-void orc_rk1(const orc_grid_t *g, const double *u, const double *v, const double *w,
-             double *u1, double *v1, double *w1, double u_crf)
+/* b: NULL, or a body force (three haloed fields) added to the right-hand
+ * side, f := (visc - conv) + b: an extension for the adjoint checks, not in
+ * the reference. */
+static void rk1_impl(const orc_grid_t *g, const double *u, const double *v, const double *w,
+                     double *u1, double *v1, double *w1, double u_crf, const double *const *b)
 { /* mod_calcVel.f90:52-146: u1 = u + dt*(visc - conv) */
     const double dt = g->dt;
 #pragma omp parallel for schedule(static) NT
@@ -720,14 +723,19 @@ void orc_rk1(const orc_grid_t *g, const double *u, const double *v, const double
                 double fu, fv, fw;
                 mom_rhs(g, u, v, w, i, j, k, u_crf, &fu, &fv, &fw);
                 size_t e = IX(g, i, j, k);
+                if (b) {
+                    fu = fu + b[0][e];
+                    fv = fv + b[1][e];
+                    fw = fw + b[2][e];
+                }
                 u1[e] = u[e] + dt * fu;
                 v1[e] = v[e] + dt * fv;
                 w1[e] = w[e] + dt * fw;
             }
 }
 
-void orc_rk2(const orc_grid_t *g, double *u, double *v, double *w,
-             const double *u1, const double *v1, const double *w1, double u_crf)
+static void rk2_impl(const orc_grid_t *g, double *u, double *v, double *w, const double *u1,
+                     const double *v1, const double *w1, double u_crf, const double *const *b)
 { /* mod_calcVel.f90:240-334: u = (u1 + dt*(visc - conv) + u)*0.5 */
     const double dt = g->dt;
 #pragma omp parallel for schedule(static) NT
@@ -737,10 +745,27 @@ void orc_rk2(const orc_grid_t *g, double *u, double *v, double *w,
                 double fu, fv, fw;
                 mom_rhs(g, u1, v1, w1, i, j, k, u_crf, &fu, &fv, &fw);
                 size_t e = IX(g, i, j, k);
+                if (b) {
+                    fu = fu + b[0][e];
+                    fv = fv + b[1][e];
+                    fw = fw + b[2][e];
+                }
                 u[e] = (u1[e] + dt * fu + u[e]) * 0.5;
                 v[e] = (v1[e] + dt * fv + v[e]) * 0.5;
                 w[e] = (w1[e] + dt * fw + w[e]) * 0.5;
             }
+}
+
+void orc_rk1(const orc_grid_t *g, const double *u, const double *v, const double *w,
+             double *u1, double *v1, double *w1, double u_crf)
+{
+    rk1_impl(g, u, v, w, u1, v1, w1, u_crf, NULL);
+}
+
+void orc_rk2(const orc_grid_t *g, double *u, double *v, double *w,
+             const double *u1, const double *v1, const double *w1, double u_crf)
+{
+    rk2_impl(g, u, v, w, u1, v1, w1, u_crf, NULL);
 }
 
 void orc_calc_rhs(const orc_grid_t *g, const double *u, const double *v,
@@ -890,6 +915,24 @@ double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf)
         tau[s] = t / g->nx / g->ny;
     }
     return 0.5 * (tau[0] + tau[1]);
+}
+
+void orc_step_body(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
+                   double *u1, double *v1, double *w1, double *p, double u_crf,
+                   const int is_forced[3], const double vel_force[3], const double *bu,
+                   const double *bv, const double *bw)
+{ /* orc_step with a body force in both RK stages (not in the reference) */
+    const double *b[3] = {bu, bv, bw};
+    rk1_impl(g, u, v, w, u1, v1, w1, u_crf, b);
+    orc_bc_vel(g, u1, v1, w1, u_crf);
+    rk2_impl(g, u, v, w, u1, v1, w1, u_crf, b);
+    orc_bc_vel(g, u, v, w, u_crf);
+    orc_calc_rhs(g, u, v, w, p);
+    orc_ppe_solve(ppe, p);
+    orc_bc_p(g, p);
+    orc_correct_vel(g, p, u, v, w);
+    orc_force_vel(g, is_forced, vel_force, u, v, w);
+    orc_bc_vel(g, u, v, w, u_crf);
 }
 
 void orc_step(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,

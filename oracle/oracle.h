@@ -101,6 +101,12 @@ void orc_bc_p(const orc_grid_t *g, double *p);
 double orc_max_div(const orc_grid_t *g, const double *u, const double *v,
                    const double *w, int ijk[3]);
 double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf);
+/* orc_step with a body force (bu, bv, bw) added to the momentum right-hand
+ * side of both RK stages (for the adjoint checks; not in the reference) */
+void orc_step_body(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
+                   double *u1, double *v1, double *w1, double *p, double u_crf,
+                   const int is_forced[3], const double vel_force[3], const double *bu,
+                   const double *bv, const double *bw);
 /* One full time step in the order of src/main.f90:137-208. */
 void orc_step(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
               double *u1, double *v1, double *w1, double *p, double u_crf,

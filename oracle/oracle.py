@@ -75,6 +75,8 @@ def lib():
         "orc_wall_shear": (C.c_double, [gp, _dp, C.c_double]),
         "orc_step": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                             C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "orc_step_body": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double), _dp, _dp, _dp]),
         "orc_set_threads": (None, [C.c_int]),
         "orc_get_threads": (C.c_int, []),
     }
@@ -246,6 +248,13 @@ def step(G, ppe: OraclePPE, u, v, w, u1, v1, w1, p, u_crf=0.0,
          is_forced=(True, False, False), vel_force=(1.0, 0.0, 0.0)):
     lib().orc_step(G.ref(), ppe.h, u, v, w, u1, v1, w1, p, u_crf, _iv(is_forced),
                    _fv(vel_force))
+
+
+def step_body(G, ppe: OraclePPE, u, v, w, u1, v1, w1, p, bu, bv, bw, u_crf=0.0,
+              is_forced=(True, False, False), vel_force=(1.0, 0.0, 0.0)):
+    """step() with a body force in both RK stages (adjoint checks only)."""
+    lib().orc_step_body(G.ref(), ppe.h, u, v, w, u1, v1, w1, p, u_crf, _iv(is_forced),
+                        _fv(vel_force), bu, bv, bw)
 
 
 def set_threads(n: int) -> None:

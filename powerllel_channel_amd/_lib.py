@@ -64,6 +64,21 @@ _SIGS = {
     "pll_channel_mean_h": (_ip, [_vp, _dp, C.POINTER(_d), _vp]),
     "pll_channel_max_div": (_ip, [_vp, _dp, _dp, _dp, C.POINTER(_d), C.POINTER(_ip), _vp]),
     "pll_channel_step": (_ip, [_vp, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "pll_channel_step_body": (_ip, [_vp, _vp] + [_dp] * 10 + [_vp]),
+    "pll_channel_wall_shear": (_ip, [_vp, _dp, C.POINTER(_d), _vp]),
+    "pll_channel_bc_vel_adj": (_ip, [_vp, _dp, _dp, _dp, _vp]),
+    "pll_channel_bc_p_adj": (_ip, [_vp, _dp, _vp]),
+    "pll_channel_force_vel_adj": (_ip, [_vp, _dp, _dp, _dp, _vp]),
+    "pll_channel_correct_vel_adj": (_ip, [_vp, _dp, _dp, _dp, _dp, _vp]),
+    "pll_channel_calc_rhs_adj": (_ip, [_vp, _dp, _dp, _dp, _dp, _vp]),
+    "pll_channel_rk_jvp": (_ip, [_vp] + [_dp] * 9 + [_vp]),
+    "pll_channel_rk_vjp": (_ip, [_vp] + [_dp] * 9 + [_d, _d, _d, _vp]),
+    "pll_ppe_solve_adjoint": (_ip, [_vp, _dp, _vp]),
+    "pll_adjoint_create": (_ip, [C.POINTER(_vp), _vp, _vp, _ip, _ip]),
+    "pll_adjoint_forward": (_ip, [_vp] + [_dp] * 6 + [C.POINTER(_d), _vp]),
+    "pll_adjoint_backward": (_ip, [_vp, _dp, _dp, _dp, _vp]),
+    "pll_adjoint_device_bytes": (C.c_size_t, [_vp]),
+    "pll_adjoint_destroy": (_ip, [_vp]),
 }
 
 

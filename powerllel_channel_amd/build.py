@@ -27,10 +27,12 @@ HIP_SOURCES = {
     "fft_kernels.hip": [],
     "trid_kernels.hip": [],
     "resid_kernels.hip": [],
+    # dual-number derivatives: let zero tangents fold away (0 * x = 0)
+    "adjoint_kernels.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
     # restates the reference's Fortran arithmetic: no fused multiply-add
     "step_kernels.hip": ["-ffp-contract=off"],
 }
-C_SOURCES = ["ppe.c", "channel.c"]
+C_SOURCES = ["ppe.c", "channel.c", "adjoint.c"]
 
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",

@@ -98,6 +98,57 @@ class Channel:
         check(lib().pll_channel_step(self._h, ppe.handle, _p(u), _p(v), _p(w), _p(u1), _p(v1),
                                      _p(w1), _p(p), _stream_ptr(stream)))
 
+    def step_body(self, ppe: PoissonSolver, u, v, w, u1, v1, w1, p, bu, bv, bw, stream=None):
+        """step() with a body force (bu, bv, bw) added to the momentum
+        right-hand side of both RK stages (not part of the reference)."""
+        self._chk(u, v, w, u1, v1, w1, p, bu, bv, bw)
+        check(lib().pll_channel_step_body(self._h, ppe.handle, _p(u), _p(v), _p(w), _p(u1),
+                                          _p(v1), _p(w1), _p(p), _p(bu), _p(bv), _p(bw),
+                                          _stream_ptr(stream)))
+
+    def wall_shear(self, u, stream=None) -> float:
+        """Wall shear of mod_monitor.f90:225-258, averaged over both walls."""
+        self._chk(u)
+        out = C.c_double()
+        check(lib().pll_channel_wall_shear(self._h, _p(u), C.byref(out), _stream_ptr(stream)))
+        return out.value
+
+    # -- transposes of the step's linear(ised) pieces (adjoint) --
+    def bc_vel_adj(self, lu, lv, lw, stream=None):
+        self._chk(lu, lv, lw)
+        check(lib().pll_channel_bc_vel_adj(self._h, _p(lu), _p(lv), _p(lw), _stream_ptr(stream)))
+
+    def bc_p_adj(self, lp, stream=None):
+        self._chk(lp)
+        check(lib().pll_channel_bc_p_adj(self._h, _p(lp), _stream_ptr(stream)))
+
+    def force_vel_adj(self, lu, lv, lw, stream=None):
+        self._chk(lu, lv, lw)
+        check(lib().pll_channel_force_vel_adj(self._h, _p(lu), _p(lv), _p(lw),
+                                              _stream_ptr(stream)))
+
+    def correct_vel_adj(self, lu, lv, lw, lp, stream=None):
+        self._chk(lu, lv, lw, lp)
+        check(lib().pll_channel_correct_vel_adj(self._h, _p(lu), _p(lv), _p(lw), _p(lp),
+                                                _stream_ptr(stream)))
+
+    def calc_rhs_adj(self, lr, lu, lv, lw, stream=None):
+        self._chk(lr, lu, lv, lw)
+        check(lib().pll_channel_calc_rhs_adj(self._h, _p(lr), _p(lu), _p(lv), _p(lw),
+                                             _stream_ptr(stream)))
+
+    def rk_jvp(self, q, dq, out, stream=None):
+        """out = J(q) dq on the interior (J: Jacobian of the momentum RHS)."""
+        self._chk(*q, *dq, *out)
+        check(lib().pll_channel_rk_jvp(self._h, *[_p(t) for t in (*q, *dq, *out)],
+                                       _stream_ptr(stream)))
+
+    def rk_vjp(self, q, l, out, alpha=1.0, a0=0.0, beta=0.0, stream=None):
+        """out = beta out + a0 l (interior) + alpha J(q)^T l on every cell."""
+        self._chk(*q, *l, *out)
+        check(lib().pll_channel_rk_vjp(self._h, *[_p(t) for t in (*q, *l, *out)], float(alpha),
+                                       float(a0), float(beta), _stream_ptr(stream)))
+
     def close(self):
         if getattr(self, "_h", None) is not None:
             lib().pll_channel_destroy(self._h)

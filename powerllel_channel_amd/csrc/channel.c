@@ -28,8 +28,10 @@ struct pll_channel {
     double *zarr;      /* device: dzf | dzf_inv | dzc_inv | dzflzi, each nz+2 */
     double *part;      /* device: ny*nz partials */
     long long *pidx;   /* device: ny*nz indices  */
-    double *scal;      /* device: 4 scalars (mean, max value)  */
+    double *scal;      /* device: 8 scalars (mean, max value, fused-step means at 4, 5) */
     long long *sidx;   /* device: 1 index */
+    double *dzf_h;     /* host copy of dzf_global (nz+2) */
+    double lx, ly, lz;
 };
 
 static int kl(int rc, const char *what)
@@ -47,6 +49,7 @@ int pll_channel_destroy(pll_channel *c)
     if (c->pidx) hipFree(c->pidx);
     if (c->scal) hipFree(c->scal);
     if (c->sidx) hipFree(c->sidx);
+    free(c->dzf_h);
     free(c);
     return PLL_OK;
 }
@@ -96,11 +99,13 @@ int pll_channel_create(pll_channel **out, int nx, int ny, int nz, double lx, dou
         c->vel_force[d] = vel_force ? vel_force[d] : (d == 0 ? 1.0 : 0.0);
     }
     hipError_t e;
-    const size_t nrow = (size_t)ny * nz;
+    size_t nrow = (size_t)ny * nz;
+    const size_t nfused = 2 * (size_t)pll_rk_blocks(&c->g); /* fused-step Mean_h partials */
+    if (nfused > nrow) nrow = nfused;
     if ((e = hipMalloc((void **)&c->zarr, sizeof(double) * 4 * (size_t)n2)) != hipSuccess ||
         (e = hipMalloc((void **)&c->part, sizeof(double) * nrow)) != hipSuccess ||
         (e = hipMalloc((void **)&c->pidx, sizeof(long long) * nrow)) != hipSuccess ||
-        (e = hipMalloc((void **)&c->scal, sizeof(double) * 4)) != hipSuccess ||
+        (e = hipMalloc((void **)&c->scal, sizeof(double) * 8)) != hipSuccess ||
         (e = hipMalloc((void **)&c->sidx, sizeof(long long))) != hipSuccess ||
         (e = hipMemcpy(c->zarr, z, sizeof(double) * 4 * (size_t)n2, hipMemcpyHostToDevice)) !=
             hipSuccess) {
@@ -109,6 +114,15 @@ int pll_channel_create(pll_channel **out, int nx, int ny, int nz, double lx, dou
         return pll_set_error(PLL_ENOMEM, "device allocation/copy failed: %s", hipGetErrorString(e));
     }
     free(z);
+    c->dzf_h = (double *)malloc(sizeof(double) * (size_t)n2);
+    if (!c->dzf_h) {
+        pll_channel_destroy(c);
+        return pll_set_error(PLL_ENOMEM, "malloc failed");
+    }
+    memcpy(c->dzf_h, dzf_global, sizeof(double) * (size_t)n2);
+    c->lx = lx;
+    c->ly = ly;
+    c->lz = lz;
     c->g.dzf = c->zarr;
     c->g.dzf_inv = c->zarr + n2;
     c->g.dzc_inv = c->zarr + 2 * n2;
@@ -124,14 +138,14 @@ int pll_channel_rk1(pll_channel *c, const double *u, const double *v, const doub
                     double *u1, double *v1, double *w1, void *stream)
 {
     CHK_C(c);
-    return kl(pll_launch_rk1(&c->g, u, v, w, u1, v1, w1, (hipStream_t)stream), "rk1");
+    return kl(pll_launch_rk1(&c->g, u, v, w, u1, v1, w1, NULL, (hipStream_t)stream), "rk1");
 }
 
 int pll_channel_rk2(pll_channel *c, double *u, double *v, double *w, const double *u1,
                     const double *v1, const double *w1, void *stream)
 {
     CHK_C(c);
-    return kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, (hipStream_t)stream), "rk2");
+    return kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, NULL, (hipStream_t)stream), "rk2");
 }
 
 int pll_channel_calc_rhs(pll_channel *c, const double *u, const double *v, const double *w,
@@ -145,7 +159,8 @@ int pll_channel_correct_vel(pll_channel *c, const double *p, double *u, double *
                             void *stream)
 {
     CHK_C(c);
-    return kl(pll_launch_correct_vel(&c->g, p, u, v, w, (hipStream_t)stream), "correct_vel");
+    return kl(pll_launch_correct_vel(&c->g, p, u, v, w, 0, 0.0, 0.0, NULL, (hipStream_t)stream),
+              "correct_vel");
 }
 
 static int mean_dev(pll_channel *c, const double *q, hipStream_t s)
@@ -223,20 +238,190 @@ int pll_channel_max_div(pll_channel *c, const double *u, const double *v, const 
     return PLL_OK;
 }
 
-int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v, double *w, double *u1,
-                     double *v1, double *w1, double *p, void *stream)
+static int step_impl(pll_channel *c, pll_ppe *ppe, double *u, double *v, double *w, double *u1,
+                     double *v1, double *w1, double *p, const double *const *body, void *stream)
 {
     CHK_C(c);
     if (!ppe) return pll_set_error(PLL_ESTATE, "PPE handle is NULL");
+    hipStream_t s = (hipStream_t)stream;
     int rc;
-    if ((rc = pll_channel_rk1(c, u, v, w, u1, v1, w1, stream))) return rc;
+    /* Forcing of u and/or v is fused: Mean_h is reduced inside RK2 and the
+     * shift applied inside correctVel.  correctVel changes u by
+     * -(p_{i+1} - p_i) dt/dx, whose sum over a periodic x row telescopes to
+     * zero (and likewise v in y), so Mean_h of the RK2 output equals Mean_h
+     * of the corrected field in exact arithmetic (they differ by roundoff in
+     * the summation).  A forced w keeps the reference's separate passes. */
+    const int fmask = (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0);
+    const int fuse = fmask && !c->is_forced[2];
+    const int nb = pll_rk_blocks(&c->g);
+    if ((rc = kl(pll_launch_rk1(&c->g, u, v, w, u1, v1, w1, body, s), "rk1"))) return rc;
     if ((rc = pll_channel_bc_vel(c, u1, v1, w1, stream))) return rc;
-    if ((rc = pll_channel_rk2(c, u, v, w, u1, v1, w1, stream))) return rc;
+    if (fuse) {
+        if ((rc = kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, c->part, fmask, body, s), "rk2")))
+            return rc;
+        for (int d = 0; d < 2; d++)
+            if ((fmask >> d) & 1)
+                if ((rc = kl(pll_launch_mean_final(nb, c->part + (size_t)d * nb, c->scal + 4 + d,
+                                                   c->g.nx, c->g.ny, s),
+                             "mean final")))
+                    return rc;
+    } else if ((rc = kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, body, s), "rk2"))) {
+        return rc;
+    }
     if ((rc = pll_channel_bc_vel(c, u, v, w, stream))) return rc;
-    if ((rc = pll_channel_calc_rhs(c, u, v, w, p, stream))) return rc;
-    if ((rc = pll_ppe_solve(ppe, p, stream))) return rc;
+    if (pll_ppe_refinement(ppe) < 0) {
+        /* calcRHS fused into the solve's x forward transform: the right-hand
+         * side is never stored (mod_calcRHS.f90:28-30, same arithmetic) */
+        const double dti = 1.0 / c->g.dt;
+        pll_div_src q = {u, v, w, c->g.dzf_inv, dti, dti * c->g.dx_inv, dti * c->g.dy_inv};
+        if ((rc = pll_ppe_solve_div(ppe, &q, p, stream))) return rc;
+    } else {
+        if ((rc = pll_channel_calc_rhs(c, u, v, w, p, stream))) return rc;
+        if ((rc = pll_ppe_solve(ppe, p, stream))) return rc;
+    }
     if ((rc = pll_channel_bc_p(c, p, stream))) return rc;
-    if ((rc = pll_channel_correct_vel(c, p, u, v, w, stream))) return rc;
-    if ((rc = pll_channel_force_vel(c, u, v, w, stream))) return rc;
+    if (fuse) {
+        if ((rc = kl(pll_launch_correct_vel(&c->g, p, u, v, w, fmask, c->vel_force[0],
+                                            c->vel_force[1], c->scal + 4, s),
+                     "correct_vel")))
+            return rc;
+    } else {
+        if ((rc = pll_channel_correct_vel(c, p, u, v, w, stream))) return rc;
+        if ((rc = pll_channel_force_vel(c, u, v, w, stream))) return rc;
+    }
     return pll_channel_bc_vel(c, u, v, w, stream);
+}
+
+int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v, double *w, double *u1,
+                     double *v1, double *w1, double *p, void *stream)
+{
+    return step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream);
+}
+
+int pll_channel_step_body_internal(pll_channel *c, pll_ppe *ppe, double *u, double *v,
+                                   double *w, double *u1, double *v1, double *w1, double *p,
+                                   const double *const *body, void *stream)
+{
+    return step_impl(c, ppe, u, v, w, u1, v1, w1, p, body, stream);
+}
+
+int pll_channel_step_body(pll_channel *c, pll_ppe *ppe, double *u, double *v, double *w,
+                          double *u1, double *v1, double *w1, double *p, const double *fu,
+                          const double *fv, const double *fw, void *stream)
+{
+    if (!fu || !fv || !fw) return pll_set_error(PLL_EINVAL, "body force field is NULL");
+    const double *body[3] = {fu, fv, fw};
+    return step_impl(c, ppe, u, v, w, u1, v1, w1, p, body, stream);
+}
+
+/* ------------------------------------------------------------------ adjoint pieces */
+const pll_grid_dev *pll_channel_grid(const pll_channel *c) { return c ? &c->g : NULL; }
+
+int pll_channel_forcing(const pll_channel *c, int is_forced[3], double vel_force[3])
+{
+    CHK_C(c);
+    for (int d = 0; d < 3; d++) {
+        if (is_forced) is_forced[d] = c->is_forced[d];
+        if (vel_force) vel_force[d] = c->vel_force[d];
+    }
+    return PLL_OK;
+}
+
+const double *pll_channel_dzf_host(const pll_channel *c) { return c ? c->dzf_h : NULL; }
+
+int pll_channel_bc_vel_adj(pll_channel *c, double *lu, double *lv, double *lw, void *stream)
+{
+    CHK_C(c);
+    if (!lu || !lv || !lw) return pll_set_error(PLL_EINVAL, "NULL field");
+    return kl(pll_launch_bc_vel_adj(&c->g, lu, lv, lw, (hipStream_t)stream), "bc_vel adjoint");
+}
+
+int pll_channel_bc_p_adj(pll_channel *c, double *lp, void *stream)
+{
+    CHK_C(c);
+    if (!lp) return pll_set_error(PLL_EINVAL, "NULL field");
+    return kl(pll_launch_bc_p_adj(&c->g, lp, (hipStream_t)stream), "bc_p adjoint");
+}
+
+int pll_channel_correct_vel_adj(pll_channel *c, const double *lu, const double *lv,
+                                const double *lw, double *lp, void *stream)
+{
+    CHK_C(c);
+    if (!lu || !lv || !lw || !lp) return pll_set_error(PLL_EINVAL, "NULL field");
+    return kl(pll_launch_correct_adj(&c->g, lu, lv, lw, lp, (hipStream_t)stream),
+              "correct_vel adjoint");
+}
+
+int pll_channel_calc_rhs_adj(pll_channel *c, const double *lr, double *lu, double *lv, double *lw,
+                             void *stream)
+{
+    CHK_C(c);
+    if (!lr || !lu || !lv || !lw) return pll_set_error(PLL_EINVAL, "NULL field");
+    return kl(pll_launch_rhs_adj(&c->g, lr, lu, lv, lw, (hipStream_t)stream), "calc_rhs adjoint");
+}
+
+int pll_channel_force_vel_adj(pll_channel *c, double *lu, double *lv, double *lw, void *stream)
+{
+    CHK_C(c);
+    if (!lu || !lv || !lw) return pll_set_error(PLL_EINVAL, "NULL field");
+    double *q3[3] = {lu, lv, lw};
+    for (int d = 0; d < 3; d++) {
+        if (!c->is_forced[d]) continue;
+        int rc;
+        if ((rc = kl(pll_launch_force_adj(&c->g, q3[d], c->part, c->scal + 6,
+                                          (hipStream_t)stream),
+                     "force_vel adjoint")))
+            return rc;
+    }
+    return PLL_OK;
+}
+
+int pll_channel_rk_jvp(pll_channel *c, const double *qu, const double *qv, const double *qw,
+                       const double *du, const double *dv, const double *dw, double *ou,
+                       double *ov, double *ow, void *stream)
+{
+    CHK_C(c);
+    const double *q[3] = {qu, qv, qw}, *d[3] = {du, dv, dw};
+    double *o[3] = {ou, ov, ow};
+    for (int i = 0; i < 3; i++)
+        if (!q[i] || !d[i] || !o[i]) return pll_set_error(PLL_EINVAL, "NULL field");
+    return kl(pll_launch_jvp(&c->g, q, d, o, (hipStream_t)stream), "rk jvp");
+}
+
+int pll_channel_rk_vjp(pll_channel *c, const double *qu, const double *qv, const double *qw,
+                       const double *lu, const double *lv, const double *lw, double *ou,
+                       double *ov, double *ow, double alpha, double a0, double beta,
+                       void *stream)
+{
+    CHK_C(c);
+    const double *q[3] = {qu, qv, qw}, *l[3] = {lu, lv, lw};
+    double *o[3] = {ou, ov, ow};
+    for (int i = 0; i < 3; i++)
+        if (!q[i] || !l[i] || !o[i]) return pll_set_error(PLL_EINVAL, "NULL field");
+    return kl(pll_launch_jtv(&c->g, q, l, o, alpha, a0, beta, (hipStream_t)stream), "rk vjp");
+}
+
+/* mod_monitor.f90:225-258 (calcAndWriteSkf) at both walls, averaged:
+ * tau = re_inv (mean_ij u(i,j,kw) + u_crf) / (dzf(kw) / 2). */
+int pll_channel_wall_shear(pll_channel *c, const double *u, double *tau, void *stream)
+{
+    CHK_C(c);
+    if (!u || !tau) return pll_set_error(PLL_EINVAL, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if ((rc = kl(pll_launch_wall_sum(&c->g, u, c->part, c->scal + 6, s), "wall shear")))
+        return rc;
+    double sums[2];
+    HIPCHK(hipMemcpyAsync(sums, c->scal + 6, sizeof sums, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *tau = pll_wall_shear_from_sums(c, sums);
+    return PLL_OK;
+}
+
+double pll_wall_shear_from_sums(const pll_channel *c, const double sums[2])
+{
+    const double nxy = (double)c->g.nx * c->g.ny;
+    const double tb = c->g.re_inv * (sums[0] / nxy + c->g.u_crf) / (0.5 * c->dzf_h[1]);
+    const double tt = c->g.re_inv * (sums[1] / nxy + c->g.u_crf) / (0.5 * c->dzf_h[c->g.nz]);
+    return 0.5 * (tb + tt);
 }

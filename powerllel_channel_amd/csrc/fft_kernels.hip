@@ -338,6 +338,29 @@ struct SrcDense32 {  // dense FP32 rows (the residual of a refinement sweep)
     }
 };
 
+struct SrcDiv {  // PPE right-hand side formed from the velocity on the fly
+    pll_div_src q;
+    long long off1, off2;
+    template <typename V>
+    __device__ __forceinline__ V load(long long, long long j, long long k, int m, int) const
+    {
+        // mod_calcRHS.f90:28-30, same operation order as the calcRHS kernel:
+        //   rhs = (u - u_{i-1}) dtidxi + (v - v_{j-1}) dtidyi + (w - w_{k-1}) dti dzf_inv_k
+#pragma clang fp contract(off)
+        const long long e = (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m;
+        const d2u uc = *reinterpret_cast<const d2u *>(q.u + e);
+        const double um = q.u[e - 1];
+        const d2u vc = *reinterpret_cast<const d2u *>(q.v + e);
+        const d2u vs = *reinterpret_cast<const d2u *>(q.v + e - off2);
+        const d2u wc = *reinterpret_cast<const d2u *>(q.w + e);
+        const d2u wb = *reinterpret_cast<const d2u *>(q.w + e - off1);
+        const double dzi = q.dzf_inv[k + 1];
+        const double r0 = (uc.x - um) * q.dtidxi + (vc.x - vs.x) * q.dtidyi + (wc.x - wb.x) * q.dti * dzi;
+        const double r1 = (uc.y - uc.x) * q.dtidxi + (vc.y - vs.y) * q.dtidyi + (wc.y - wb.y) * q.dti * dzi;
+        return mk<V>((typename Sc<V>::T)r0, (typename Sc<V>::T)r1);
+    }
+};
+
 // Destinations of the x backward transform; (a, b) = normalised (x_2m, x_2m+1).
 struct DstHalo64 {  // haloed FP64 array interior := value
     double *p;
@@ -748,6 +771,15 @@ extern "C" int pll_fft_ct_plan(int L, int prefer_E, pll_fftplan *p)
 extern "C" int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s)
 {
     const SrcHalo64 src{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s);
+    });
+}
+
+extern "C" int pll_launch_xfwd_div(const pll_ppe_dev *d, const pll_div_src *q, void *W,
+                                   hipStream_t s)
+{
+    const SrcDiv src{*q, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s);
     });

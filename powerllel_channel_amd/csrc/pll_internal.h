@@ -60,6 +60,15 @@ typedef struct {
 
 int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s);
 
+/* Velocity source of the x forward transform: the PPE right-hand side of
+ * calcRHS formed in the transform's load (haloed u, v, w; dzf_inv with the
+ * ghost entries, indexed by the haloed k). */
+typedef struct {
+    const double *u, *v, *w, *dzf_inv;
+    double dti, dtidxi, dtidyi;
+} pll_div_src;
+int pll_launch_xfwd_div(const pll_ppe_dev *d, const pll_div_src *q, void *W, hipStream_t s);
+
 /* FP32 solve of the refined form.  W32 is the spectral array in FP32 (N
  * floats, same layout as W).  x forward sources and x backward destinations: */
 enum { PLL_SRC_HALO64 = 0, PLL_SRC_DENSE32 = 1 };
@@ -104,14 +113,24 @@ typedef struct {
     const double *dzf, *dzf_inv, *dzc_inv, *dzflzi; /* [nz+2] */
 } pll_grid_dev;
 
+/* body: NULL, or three device fields (haloed layout) added to the momentum
+ * right-hand side (a body force; not part of the reference's step). */
 int pll_launch_rk1(const pll_grid_dev *g, const double *u, const double *v,
-                   const double *w, double *u1, double *v1, double *w1, hipStream_t s);
+                   const double *w, double *u1, double *v1, double *w1,
+                   const double *const *body, hipStream_t s);
+/* mpart non-NULL and fmask != 0: also the Mean_h partial sums of the new u
+ * (fmask bit 0) and v (bit 1), pll_rk_blocks(g) of each, u's first. */
 int pll_launch_rk2(const pll_grid_dev *g, double *u, double *v, double *w,
-                   const double *u1, const double *v1, const double *w1, hipStream_t s);
+                   const double *u1, const double *v1, const double *w1, double *mpart,
+                   int fmask, const double *const *body, hipStream_t s);
+int pll_rk_blocks(const pll_grid_dev *g);
 int pll_launch_calc_rhs(const pll_grid_dev *g, const double *u, const double *v,
                         const double *w, double *p, hipStream_t s);
+/* mean non-NULL and fmask != 0: then also q += vel_force - mean[d] for the
+ * forced components (bit 0 u with vfu, bit 1 v with vfv). */
 int pll_launch_correct_vel(const pll_grid_dev *g, const double *p, double *u,
-                           double *v, double *w, hipStream_t s);
+                           double *v, double *w, int fmask, double vfu, double vfv,
+                           const double *mean, hipStream_t s);
 /* partial sums of q*dzflzi per (j,k) row into part[ny*nz]; then a single
  * workgroup reduces them to out[0] = sum / nx / ny (Mean_h). */
 int pll_launch_mean_partials(const pll_grid_dev *g, const double *q, double *part,
@@ -130,6 +149,56 @@ int pll_launch_maxdiv_partials(const pll_grid_dev *g, const double *u,
                                long long *pidx, hipStream_t s);
 int pll_launch_maxdiv_final(int n, const double *pval, const long long *pidx,
                             double *oval, long long *oidx, hipStream_t s);
+
+/* ---- adjoint pieces (adjoint_kernels.hip); every array is haloed ---- */
+/* o = beta o + a0 l [interior] + alpha J_f(q)^T l, over all cells; l read on
+ * the interior (J_f: Jacobian of the momentum right-hand side). */
+int pll_launch_jtv(const pll_grid_dev *g, const double *const q[3], const double *const l[3],
+                   double *const o[3], double alpha, double a0, double beta, hipStream_t s);
+/* o = J_f(q) dq on the interior. */
+int pll_launch_jvp(const pll_grid_dev *g, const double *const q[3], const double *const dq[3],
+                   double *const o[3], hipStream_t s);
+/* transposes of the boundary fills: ghost adjoints folded into the interior
+ * values they were made from, ghosts cleared */
+int pll_launch_bc_vel_adj(const pll_grid_dev *g, double *lu, double *lv, double *lw,
+                          hipStream_t s);
+int pll_launch_bc_p_adj(const pll_grid_dev *g, double *lp, hipStream_t s);
+/* lp := transpose of correctVel's pressure term applied to (lu, lv, lw) */
+int pll_launch_correct_adj(const pll_grid_dev *g, const double *lu, const double *lv,
+                           const double *lw, double *lp, hipStream_t s);
+/* (lu, lv, lw) += transpose of calcRHS applied to lr (interior) */
+int pll_launch_rhs_adj(const pll_grid_dev *g, const double *lr, double *lu, double *lv,
+                       double *lw, hipStream_t s);
+/* q -= dzflzi(k) sum_interior(q) / (nx ny); part: ny*nz, total: 1 double */
+int pll_launch_force_adj(const pll_grid_dev *g, double *q, double *part, double *total,
+                         hipStream_t s);
+/* out[0], out[1] = sum over (i,j) of u at k = 1 and k = nz; part: 2 ny */
+int pll_launch_wall_sum(const pll_grid_dev *g, const double *u, double *part, double *out,
+                        hipStream_t s);
+/* lu += c_bot on the plane k = 1 and c_top on k = nz (interior) */
+int pll_launch_wall_seed(const pll_grid_dev *g, double *lu, double c_bot, double c_top,
+                         hipStream_t s);
+/* y += a x over every cell, three fields */
+int pll_launch_axpy3(const pll_grid_dev *g, double a, const double *const x[3],
+                     double *const y[3], hipStream_t s);
+
+/* Host-layer calls between ppe.c and channel.c (pll_ppe from powerllel.h). */
+struct pll_ppe;
+/* FP64 solve whose right-hand side is formed from the velocity inside the x
+ * forward transform (calcRHS fused into the solve); the solution lands in the
+ * interior of d_var as with pll_ppe_solve.  Only for the FP64 form. */
+int pll_ppe_solve_div(struct pll_ppe *h, const pll_div_src *q, double *d_var, void *stream);
+/* sweeps of the refined form that pll_ppe_solve uses (-1: FP64 solve). */
+int pll_ppe_refinement(const struct pll_ppe *h);
+struct pll_channel;
+const pll_grid_dev *pll_channel_grid(const struct pll_channel *c);
+int pll_channel_forcing(const struct pll_channel *c, int is_forced[3], double vel_force[3]);
+const double *pll_channel_dzf_host(const struct pll_channel *c);
+/* averaged wall shear from the two plane sums of u */
+double pll_wall_shear_from_sums(const struct pll_channel *c, const double sums[2]);
+int pll_channel_step_body_internal(struct pll_channel *c, struct pll_ppe *ppe, double *u,
+                                   double *v, double *w, double *u1, double *v1, double *w1,
+                                   double *p, const double *const *body, void *stream);
 
 #ifdef __cplusplus
 }

@@ -282,7 +282,8 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     const size_t npiv = (size_t)nz * nyh * (L + 1);
     double *a = (double *)malloc(sizeof(double) * (size_t)nz);
     double *c = (double *)malloc(sizeof(double) * (size_t)nz);
-    double *bz = (double *)malloc(sizeof(double) * 2 * (size_t)nz); /* bz, then row sums */
+    /* bz | row sums | transposed off-diagonals aT, cT */
+    double *bz = (double *)malloc(sizeof(double) * 4 * (size_t)nz);
     double *lx = (double *)malloc(sizeof(double) * (size_t)nx);
     double *ly = (double *)malloc(sizeof(double) * (size_t)ny);
     double *piv = (double *)malloc(sizeof(double) * npiv);
@@ -293,6 +294,12 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     }
     if ((rc = trid_coeff(nz, dzf_global, bctype_z, a, bz, c))) goto fail;
     for (int k = 0; k < nz; k++) bz[nz + k] = rowsum3(a[k], bz[k], c[k]);
+    /* transposed operator: sub-diagonal c_{k-1}, super-diagonal a_{k+1}; its
+     * pivots equal the forward ones (each uses the product a_k c_{k-1}) */
+    for (int k = 0; k < nz; k++) {
+        bz[2 * nz + k] = k > 0 ? c[k - 1] : 0.0;
+        bz[3 * nz + k] = k < nz - 1 ? a[k + 1] : 0.0;
+    }
     eigen_pp(nx, dx, lx);
     eigen_pp(ny, dy, ly);
     /* pivots: b_k = bz_k + lambda; b_k -= (a_k / b_{k-1}) c_{k-1}
@@ -315,7 +322,7 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     h->sweeps = -1;
     h->d.idx2 = 1.0 / (dx * dx);
     h->d.idy2 = 1.0 / (dy * dy);
-    if ((rc = dmalloc(h, &h->bz, sizeof(double) * 2 * (size_t)nz)) ||
+    if ((rc = dmalloc(h, &h->bz, sizeof(double) * 4 * (size_t)nz)) ||
         (rc = dmalloc(h, &h->W, sizeof(double) * N)) ||
         (rc = dmalloc(h, &h->tw_x, sizeof(double) * 2 * (size_t)nx)) ||
         (rc = dmalloc(h, &h->tw_y, sizeof(double) * 2 * (size_t)ny)) ||
@@ -336,7 +343,7 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
     H2D(h->a, a, sizeof(double) * (size_t)nz);
     H2D(h->c, c, sizeof(double) * (size_t)nz);
     H2D(h->piv, piv, sizeof(double) * npiv);
-    H2D(h->bz, bz, sizeof(double) * 2 * (size_t)nz);
+    H2D(h->bz, bz, sizeof(double) * 4 * (size_t)nz);
 #undef H2D
     h->d.tw_x = h->tw_x;
     h->d.tw_y = h->tw_y;
@@ -471,16 +478,15 @@ static int launch_rc(int rc, const char *what)
 
 int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream);
 
-int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
+/* FP64 solve; the right-hand side is the interior of d_var (q == NULL) or
+ * formed from the velocity by the x forward transform. */
+static int solve64(pll_ppe *h, const pll_div_src *q, double *d_var, hipStream_t s)
 {
-    if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
-    if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
-    if (h->sweeps >= 0) return pll_ppe_solve_refined(h, d_var, h->sweeps, stream);
-    hipStream_t s = (hipStream_t)stream;
     int rc;
     hipEvent_t *ev = h->timing ? h->ev[h->nrec % PLL_TIMING_RING] : NULL;
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    if ((rc = launch_rc(pll_launch_xfwd(&h->d, d_var, h->W, s), "x forward"))) return rc;
+    rc = q ? pll_launch_xfwd_div(&h->d, q, h->W, s) : pll_launch_xfwd(&h->d, d_var, h->W, s);
+    if ((rc = launch_rc(rc, "x forward"))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if ((rc = launch_rc(pll_launch_yfwd(&h->d, h->W, s), "y forward"))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
@@ -494,6 +500,46 @@ int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
         h->nrec++;
     }
     return PLL_OK;
+}
+
+int pll_ppe_solve(pll_ppe *h, double *d_var, void *stream)
+{
+    if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
+    if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
+    if (h->sweeps >= 0) return pll_ppe_solve_refined(h, d_var, h->sweeps, stream);
+    return solve64(h, NULL, d_var, (hipStream_t)stream);
+}
+
+int pll_ppe_solve_div(pll_ppe *h, const pll_div_src *q, double *d_var, void *stream)
+{
+    if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
+    if (!q || !q->u || !q->v || !q->w || !q->dzf_inv || !d_var)
+        return pll_set_error(PLL_EINVAL, "NULL field");
+    if (h->sweeps >= 0)
+        return pll_set_error(PLL_ESTATE, "the fused right-hand side is for the FP64 solve only");
+    return solve64(h, q, d_var, (hipStream_t)stream);
+}
+
+int pll_ppe_refinement(const pll_ppe *h) { return h ? h->sweeps : -1; }
+
+int pll_ppe_solve_adjoint(pll_ppe *h, double *d_var, void *stream)
+{
+    if (!h) return pll_set_error(PLL_ESTATE, "solver not initialised");
+    if (!d_var) return pll_set_error(PLL_EINVAL, "var is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    /* The x and y transforms and the eigenvalue scaling are orthogonal up to
+     * a common factor, so the transpose of the solve is the same pipeline
+     * with each z system transposed.  The pinned (0,0) system transposes to
+     * the same pinned form (its top row and column drop out). */
+    pll_ppe_dev dT = h->d;
+    dT.a = (const double *)h->bz + 2 * (size_t)h->nz;
+    dT.c = (const double *)h->bz + 3 * (size_t)h->nz;
+    int rc;
+    if ((rc = launch_rc(pll_launch_xfwd(&dT, d_var, h->W, s), "x forward"))) return rc;
+    if ((rc = launch_rc(pll_launch_yfwd(&dT, h->W, s), "y forward"))) return rc;
+    if ((rc = launch_rc(pll_launch_trid(&dT, h->W, s), "z solve (transposed)"))) return rc;
+    if ((rc = launch_rc(pll_launch_ybwd(&dT, h->W, s), "y backward"))) return rc;
+    return launch_rc(pll_launch_xbwd(&dT, h->W, d_var, s), "x backward");
 }
 
 /* ------------------------------------------------------------------ refined form */

@@ -12,121 +12,170 @@
 // Fields are the reference's haloed arrays f(0:nx+1,0:ny+1,0:nz+1), x fastest.
 #include <hip/hip_runtime.h>
 
+#include "mom_rhs.h"
 #include "pll_internal.h"
 
 namespace {
 
-struct G {
-    int nx, ny, nz;
-    double dx_inv, dy_inv, re_inv, dt, u_crf;
-    const double *dzf, *dzf_inv, *dzc_inv, *dzflzi;
-    long long sy, sz;  // strides of j and k
+using pll::G;
+using pll::Nb;
+using pll::make_g;
+using pll::mom_rhs;
+
+
+// RK stencils march over a chunk of k per thread: the values of plane k+1
+// that become plane k (and then k-1) at the next step stay in registers, so
+// each plane of the three fields is fetched once per chunk; x and y
+// neighbours come through the caches.  Workgroups are numbered so that each
+// of the 8 XCDs (which take workgroups round-robin) gets one contiguous run
+// of tiles: y-adjacent tiles of one k-chunk then share that XCD's L2.
+constexpr int RK_TX = 64, RK_TY = 4;
+struct Tiles {
+    int ntx, nty, nkc, kc;
+    int xcd;  // 1: remap workgroup ids per XCD (tile count divisible by 8)
 };
 
-__host__ G make_g(const pll_grid_dev *g)
+__device__ __forceinline__ void rk_tile(const Tiles &t, int &tx, int &ty, int &kc)
 {
-    G r;
-    r.nx = g->nx;
-    r.ny = g->ny;
-    r.nz = g->nz;
-    r.dx_inv = g->dx_inv;
-    r.dy_inv = g->dy_inv;
-    r.re_inv = g->re_inv;
-    r.dt = g->dt;
-    r.u_crf = g->u_crf;
-    r.dzf = g->dzf;
-    r.dzf_inv = g->dzf_inv;
-    r.dzc_inv = g->dzc_inv;
-    r.dzflzi = g->dzflzi;
-    r.sy = g->nx + 2;
-    r.sz = (long long)(g->nx + 2) * (g->ny + 2);
-    return r;
+    const int nb = t.ntx * t.nty * t.nkc;
+    int lb = blockIdx.x;
+    if (t.xcd) lb = (lb & 7) * (nb >> 3) + (lb >> 3);
+    tx = lb % t.ntx;
+    lb /= t.ntx;
+    ty = lb % t.nty;
+    kc = lb / t.nty;
 }
 
-// Momentum right-hand sides (visc - conv) of the u, v and w equations at
-// interior cell (i,j,k) for the velocity (qu,qv,qw): mod_calcVel.f90:74-138.
-__device__ __forceinline__ void mom_rhs(const G &g, const double *__restrict__ qu,
-                                        const double *__restrict__ qv,
-                                        const double *__restrict__ qw, int i, int j, int k,
-                                        double &fu, double &fv, double &fw)
+// SECOND = false: o := q + dt f(q)                  (RK1, o = u1, q = u)
+// SECOND = true:  o := (q + dt f(q) + o) * 0.5      (RK2, o = u,  q = u1)
+// MEAN (RK2 only): also the Mean_h partial sums of the new u (fmask bit 0)
+// and v (bit 1), sum of o * dzflzi(k) over the block's cells, into
+// mpart[blockIdx.x] and mpart[nblocks + blockIdx.x].
+// BODY: a body force b (haloed layout, interior read) is added to the
+// right-hand side, f := f(q) + b, in both stages.
+struct Body {
+    const double *u, *v, *w;
+};
+
+template <bool SECOND, bool MEAN, bool BODY>
+__global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
+                                                     const double *__restrict__ qv,
+                                                     const double *__restrict__ qw,
+                                                     double *__restrict__ ou,
+                                                     double *__restrict__ ov,
+                                                     double *__restrict__ ow,
+                                                     double *__restrict__ mpart, int fmask,
+                                                     Body fb)
 {
+    int tx, ty, kc;
+    rk_tile(t, tx, ty, kc);
+    const int i = tx * RK_TX + (int)(threadIdx.x % RK_TX) + 1;
+    const int j = ty * RK_TY + (int)(threadIdx.x / RK_TX) + 1;
+    double su = 0.0, sv = 0.0;
+    if (i <= g.nx && j <= g.ny) {
+        const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
+        const long long sy = g.sy, sz = g.sz;
+        long long e = i + sy * j + sz * k0;
+        Nb n;
+        double u00m = qu[e - sz], v00m = qv[e - sz], w00m = qw[e - sz];
+        double wp0m = qw[e - sz + 1], w0pm = qw[e - sz + sy];
+        double u000 = qu[e], um00 = qu[e - 1], v000 = qv[e], v0m0 = qv[e - sy], w000 = qw[e];
+        for (int k = k0; k <= k1; ++k, e += sz) {
+            n.u000 = u000; n.um00 = um00; n.u00m = u00m;
+            n.up00 = qu[e + 1]; n.u0p0 = qu[e + sy]; n.u0m0 = qu[e - sy];
+            n.ump0 = qu[e - 1 + sy]; n.u00p = qu[e + sz]; n.um0p = qu[e - 1 + sz];
+            n.v000 = v000; n.v0m0 = v0m0; n.v00m = v00m;
+            n.vp00 = qv[e + 1]; n.vm00 = qv[e - 1]; n.v0p0 = qv[e + sy];
+            n.vpm0 = qv[e + 1 - sy]; n.v00p = qv[e + sz]; n.v0mp = qv[e - sy + sz];
+            n.w000 = w000; n.w00m = w00m; n.wp0m = wp0m; n.w0pm = w0pm;
+            n.wp00 = qw[e + 1]; n.wm00 = qw[e - 1]; n.w0p0 = qw[e + sy];
+            n.w0m0 = qw[e - sy]; n.w00p = qw[e + sz];
+            double fu, fv, fw;
+            mom_rhs(g, n, k, fu, fv, fw);
+            if (BODY) {
+                fu = fu + fb.u[e];
+                fv = fv + fb.v[e];
+                fw = fw + fb.w[e];
+            }
+            if (SECOND) {
+                const double un = (n.u000 + g.dt * fu + ou[e]) * 0.5;
+                const double vn = (n.v000 + g.dt * fv + ov[e]) * 0.5;
+                ou[e] = un;
+                ov[e] = vn;
+                ow[e] = (n.w000 + g.dt * fw + ow[e]) * 0.5;
+                if (MEAN) {
+                    const double wk = g.dzflzi[k];
+                    if (fmask & 1) su = su + un * wk;
+                    if (fmask & 2) sv = sv + vn * wk;
+                }
+            } else {
+                ou[e] = n.u000 + g.dt * fu;
+                ov[e] = n.v000 + g.dt * fv;
+                ow[e] = n.w000 + g.dt * fw;
+            }
+            u00m = u000; u000 = n.u00p; um00 = n.um0p;
+            v00m = v000; v000 = n.v00p; v0m0 = n.v0mp;
+            w00m = w000; w000 = n.w00p; wp0m = n.wp00; w0pm = n.w0p0;
+        }
+    }
+    if (MEAN) {
+        __shared__ double red[2][RK_TX * RK_TY];
+        const int tt = threadIdx.x;
+        red[0][tt] = su;
+        red[1][tt] = sv;
+        __syncthreads();
+        for (int h = RK_TX * RK_TY / 2; h > 0; h >>= 1) {
+            if (tt < h) {
+                red[0][tt] += red[0][tt + h];
+                red[1][tt] += red[1][tt + h];
+            }
+            __syncthreads();
+        }
+        if (tt == 0) {
+            mpart[blockIdx.x] = red[0][0];
+            mpart[gridDim.x + blockIdx.x] = red[1][0];
+        }
+    }
+}
+
+// correctVel (mod_calcVel.f90:428-453) marching over a k chunk (p(k+1) is the
+// next step's p(k)).  FORCE: then forceVel's shift q += vel_force - mean for
+// the components in fmask (bit 0 u, bit 1 v), with the means in mean[0..1],
+// in the same two roundings as correctVel followed by forceVel.
+template <bool FORCE>
+__global__ void __launch_bounds__(RK_TX *RK_TY) k_correct(G g, Tiles t, const double *__restrict__ p,
+                                                          double *__restrict__ u,
+                                                          double *__restrict__ v,
+                                                          double *__restrict__ w, double dtdxi,
+                                                          double dtdyi, int fmask, double vfu,
+                                                          double vfv,
+                                                          const double *__restrict__ mean)
+{
+    int tx, ty, kc;
+    rk_tile(t, tx, ty, kc);
+    const int i = tx * RK_TX + (int)(threadIdx.x % RK_TX) + 1;
+    const int j = ty * RK_TY + (int)(threadIdx.x / RK_TX) + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
     const long long sy = g.sy, sz = g.sz;
-    const long long c = i + sy * j + sz * k;
-#define U(di, dj, dk) qu[c + (di) + sy * (dj) + sz * (dk)]
-#define V(di, dj, dk) qv[c + (di) + sy * (dj) + sz * (dk)]
-#define W(di, dj, dk) qw[c + (di) + sy * (dj) + sz * (dk)]
-    const double dx_inv = g.dx_inv, dy_inv = g.dy_inv, re_inv = g.re_inv, u_crf = g.u_crf;
-    const double dzf_km = g.dzf[k - 1], dzf_k = g.dzf[k], dzf_kp = g.dzf[k + 1];
-    const double dzfi_k = g.dzf_inv[k], dzfi_kp = g.dzf_inv[k + 1];
-    const double dzci_k = g.dzc_inv[k], dzci_km = g.dzc_inv[k - 1];
-    double uw, ue, us, un, ub, ut, ww, we, ws, wn, wb, wt, vw, ve, vs, vn, vb, vt;
-    double dqw, dqe, dqs, dqn, dqb, dqt, conv, visc;
-
-    ue = (U(0, 0, 0) + U(1, 0, 0));
-    uw = (U(-1, 0, 0) + U(0, 0, 0));
-    un = (U(0, 0, 0) + U(0, 1, 0));
-    us = (U(0, -1, 0) + U(0, 0, 0));
-    ut = (U(0, 0, 0) * dzf_kp + U(0, 0, 1) * dzf_k) * dzci_k;
-    ub = (U(0, 0, 0) * dzf_km + U(0, 0, -1) * dzf_k) * dzci_km;
-    vn = (V(0, 0, 0) + V(1, 0, 0));
-    vs = (V(0, -1, 0) + V(1, -1, 0));
-    wt = (W(0, 0, 0) + W(1, 0, 0));
-    wb = (W(0, 0, -1) + W(1, 0, -1));
-    dqe = (U(1, 0, 0) - U(0, 0, 0)) * dx_inv;
-    dqw = (U(0, 0, 0) - U(-1, 0, 0)) * dx_inv;
-    dqn = (U(0, 1, 0) - U(0, 0, 0)) * dy_inv;
-    dqs = (U(0, 0, 0) - U(0, -1, 0)) * dy_inv;
-    dqt = (U(0, 0, 1) - U(0, 0, 0)) * dzci_k;
-    dqb = (U(0, 0, 0) - U(0, 0, -1)) * dzci_km;
-    conv = 0.25 * ((ue * ue - uw * uw) * dx_inv + (un * vn - us * vs) * dy_inv + (ut * wt - ub * wb) * dzfi_k);
-    conv = conv + 0.5 * u_crf * (ue - uw) * dx_inv;
-    visc = ((dqe - dqw) * dx_inv + (dqn - dqs) * dy_inv + (dqt - dqb) * dzfi_k) * re_inv;
-    fu = visc - conv;
-
-    ve = (V(0, 0, 0) + V(1, 0, 0));
-    vw = (V(-1, 0, 0) + V(0, 0, 0));
-    vn = (V(0, 0, 0) + V(0, 1, 0));
-    vs = (V(0, -1, 0) + V(0, 0, 0));
-    vt = (V(0, 0, 0) * dzf_kp + V(0, 0, 1) * dzf_k) * dzci_k;
-    vb = (V(0, 0, 0) * dzf_km + V(0, 0, -1) * dzf_k) * dzci_km;
-    ue = (U(0, 0, 0) + U(0, 1, 0));
-    uw = (U(-1, 0, 0) + U(-1, 1, 0));
-    wt = (W(0, 0, 0) + W(0, 1, 0));
-    wb = (W(0, 0, -1) + W(0, 1, -1));
-    dqe = (V(1, 0, 0) - V(0, 0, 0)) * dx_inv;
-    dqw = (V(0, 0, 0) - V(-1, 0, 0)) * dx_inv;
-    dqn = (V(0, 1, 0) - V(0, 0, 0)) * dy_inv;
-    dqs = (V(0, 0, 0) - V(0, -1, 0)) * dy_inv;
-    dqt = (V(0, 0, 1) - V(0, 0, 0)) * dzci_k;
-    dqb = (V(0, 0, 0) - V(0, 0, -1)) * dzci_km;
-    conv = 0.25 * ((ue * ve - uw * vw) * dx_inv + (vn * vn - vs * vs) * dy_inv + (wt * vt - wb * vb) * dzfi_k);
-    conv = conv + 0.5 * u_crf * (ve - vw) * dx_inv;
-    visc = ((dqe - dqw) * dx_inv + (dqn - dqs) * dy_inv + (dqt - dqb) * dzfi_k) * re_inv;
-    fv = visc - conv;
-
-    we = (W(0, 0, 0) + W(1, 0, 0));
-    ww = (W(-1, 0, 0) + W(0, 0, 0));
-    wn = (W(0, 0, 0) + W(0, 1, 0));
-    ws = (W(0, -1, 0) + W(0, 0, 0));
-    wt = (W(0, 0, 0) + W(0, 0, 1));
-    wb = (W(0, 0, 0) + W(0, 0, -1));
-    ue = (U(0, 0, 0) * dzf_kp + U(0, 0, 1) * dzf_k) * dzci_k;
-    uw = (U(-1, 0, 0) * dzf_kp + U(-1, 0, 1) * dzf_k) * dzci_k;
-    vn = (V(0, 0, 0) * dzf_kp + V(0, 0, 1) * dzf_k) * dzci_k;
-    vs = (V(0, -1, 0) * dzf_kp + V(0, -1, 1) * dzf_k) * dzci_k;
-    dqe = (W(1, 0, 0) - W(0, 0, 0)) * dx_inv;
-    dqw = (W(0, 0, 0) - W(-1, 0, 0)) * dx_inv;
-    dqn = (W(0, 1, 0) - W(0, 0, 0)) * dy_inv;
-    dqs = (W(0, 0, 0) - W(0, -1, 0)) * dy_inv;
-    dqt = (W(0, 0, 1) - W(0, 0, 0)) * dzfi_kp;
-    dqb = (W(0, 0, 0) - W(0, 0, -1)) * dzfi_k;
-    conv = 0.25 * ((ue * we - uw * ww) * dx_inv + (vn * wn - vs * ws) * dy_inv + (wt * wt - wb * wb) * dzci_k);
-    conv = conv + 0.5 * u_crf * (we - ww) * dx_inv;
-    visc = ((dqe - dqw) * dx_inv + (dqn - dqs) * dy_inv + (dqt - dqb) * dzci_k) * re_inv;
-    fw = visc - conv;
-#undef U
-#undef V
-#undef W
+    long long e = i + sy * j + sz * k0;
+    double fu = 0.0, fv = 0.0;
+    if (FORCE) {
+        if (fmask & 1) fu = vfu - mean[0];
+        if (fmask & 2) fv = vfv - mean[1];
+    }
+    double pc = p[e];
+    for (int k = k0; k <= k1; ++k, e += sz) {
+        const double pz = p[e + sz];
+        double un = u[e] - (p[e + 1] - pc) * dtdxi;
+        double vn = v[e] - (p[e + sy] - pc) * dtdyi;
+        if (FORCE && (fmask & 1)) un = un + fu;
+        if (FORCE && (fmask & 2)) vn = vn + fv;
+        u[e] = un;
+        v[e] = vn;
+        w[e] = w[e] - (pz - pc) * g.dt * g.dzc_inv[k];
+        pc = pz;
+    }
 }
 
 #define CELL_IDX                                                   \
@@ -135,33 +184,6 @@ __device__ __forceinline__ void mom_rhs(const G &g, const double *__restrict__ q
     const int k = blockIdx.z + 1;                                  \
     if (i > g.nx || j > g.ny) return;                              \
     const long long e = i + g.sy * j + g.sz * k;
-
-__global__ void __launch_bounds__(256) k_rk1(G g, const double *__restrict__ u,
-                                              const double *__restrict__ v,
-                                              const double *__restrict__ w, double *__restrict__ u1,
-                                              double *__restrict__ v1, double *__restrict__ w1)
-{
-    CELL_IDX
-    double fu, fv, fw;
-    mom_rhs(g, u, v, w, i, j, k, fu, fv, fw);
-    u1[e] = u[e] + g.dt * fu;
-    v1[e] = v[e] + g.dt * fv;
-    w1[e] = w[e] + g.dt * fw;
-}
-
-__global__ void __launch_bounds__(256) k_rk2(G g, double *__restrict__ u, double *__restrict__ v,
-                                              double *__restrict__ w,
-                                              const double *__restrict__ u1,
-                                              const double *__restrict__ v1,
-                                              const double *__restrict__ w1)
-{
-    CELL_IDX
-    double fu, fv, fw;
-    mom_rhs(g, u1, v1, w1, i, j, k, fu, fv, fw);
-    u[e] = (u1[e] + g.dt * fu + u[e]) * 0.5;
-    v[e] = (v1[e] + g.dt * fv + v[e]) * 0.5;
-    w[e] = (w1[e] + g.dt * fw + w[e]) * 0.5;
-}
 
 __global__ void __launch_bounds__(256) k_calc_rhs(G g, const double *__restrict__ u,
                                                    const double *__restrict__ v,
@@ -172,19 +194,6 @@ __global__ void __launch_bounds__(256) k_calc_rhs(G g, const double *__restrict_
     CELL_IDX
     rhs[e] = (u[e] - u[e - 1]) * dtidxi + (v[e] - v[e - g.sy]) * dtidyi +
              (w[e] - w[e - g.sz]) * dti * g.dzf_inv[k];
-}
-
-__global__ void __launch_bounds__(256) k_correct_vel(G g, const double *__restrict__ p,
-                                                      double *__restrict__ u,
-                                                      double *__restrict__ v,
-                                                      double *__restrict__ w, double dtdxi,
-                                                      double dtdyi)
-{
-    CELL_IDX
-    const double pc = p[e];
-    u[e] = u[e] - (p[e + 1] - pc) * dtdxi;
-    v[e] = v[e] - (p[e + g.sy] - pc) * dtdyi;
-    w[e] = w[e] - (p[e + g.sz] - pc) * g.dt * g.dzc_inv[k];
 }
 
 // One workgroup per interior (j,k) row: sum_i q(i,j,k)*dzflzi(k).
@@ -404,21 +413,64 @@ __global__ void __launch_bounds__(1024) k_maxdiv_final(int n, const double *__re
 inline dim3 cell_grid(const G &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -3; }
 
+Tiles rk_tiles(const G &g)
+{
+    Tiles t;
+    t.kc = 32;
+    t.ntx = (g.nx + RK_TX - 1) / RK_TX;
+    t.nty = (g.ny + RK_TY - 1) / RK_TY;
+    t.nkc = (g.nz + t.kc - 1) / t.kc;
+    t.xcd = ((long long)t.ntx * t.nty * t.nkc) % 8 == 0;
+    return t;
+}
+
 }  // namespace
 
+extern "C" int pll_rk_blocks(const pll_grid_dev *gd)
+{
+    const Tiles t = rk_tiles(make_g(gd));
+    return t.ntx * t.nty * t.nkc;
+}
+
 extern "C" int pll_launch_rk1(const pll_grid_dev *gd, const double *u, const double *v,
-                              const double *w, double *u1, double *v1, double *w1, hipStream_t s)
+                              const double *w, double *u1, double *v1, double *w1,
+                              const double *const *body, hipStream_t s)
 {
     const G g = make_g(gd);
-    hipLaunchKernelGGL(k_rk1, cell_grid(g), dim3(64, 4, 1), 0, s, g, u, v, w, u1, v1, w1);
+    const Tiles t = rk_tiles(g);
+    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
+    if (body) {
+        const Body fb{body[0], body[1], body[2]};
+        hipLaunchKernelGGL((k_rk<false, false, true>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
+                           w1, nullptr, 0, fb);
+    } else {
+        hipLaunchKernelGGL((k_rk<false, false, false>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
+                           w1, nullptr, 0, Body{nullptr, nullptr, nullptr});
+    }
     return ok();
 }
 
 extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, double *w,
-                              const double *u1, const double *v1, const double *w1, hipStream_t s)
+                              const double *u1, const double *v1, const double *w1,
+                              double *mpart, int fmask, const double *const *body, hipStream_t s)
 {
     const G g = make_g(gd);
-    hipLaunchKernelGGL(k_rk2, cell_grid(g), dim3(64, 4, 1), 0, s, g, u, v, w, u1, v1, w1);
+    const Tiles t = rk_tiles(g);
+    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
+    const bool mean = mpart && fmask;
+    const Body fb = body ? Body{body[0], body[1], body[2]} : Body{nullptr, nullptr, nullptr};
+    if (mean && body)
+        hipLaunchKernelGGL((k_rk<true, true, true>), grid, blk, 0, s, g, t, u1, v1, w1, u, v, w,
+                           mpart, fmask, fb);
+    else if (mean)
+        hipLaunchKernelGGL((k_rk<true, true, false>), grid, blk, 0, s, g, t, u1, v1, w1, u, v, w,
+                           mpart, fmask, fb);
+    else if (body)
+        hipLaunchKernelGGL((k_rk<true, false, true>), grid, blk, 0, s, g, t, u1, v1, w1, u, v, w,
+                           nullptr, 0, fb);
+    else
+        hipLaunchKernelGGL((k_rk<true, false, false>), grid, blk, 0, s, g, t, u1, v1, w1, u, v,
+                           w, nullptr, 0, fb);
     return ok();
 }
 
@@ -434,12 +486,19 @@ extern "C" int pll_launch_calc_rhs(const pll_grid_dev *gd, const double *u, cons
 }
 
 extern "C" int pll_launch_correct_vel(const pll_grid_dev *gd, const double *p, double *u,
-                                      double *v, double *w, hipStream_t s)
+                                      double *v, double *w, int fmask, double vfu, double vfv,
+                                      const double *mean, hipStream_t s)
 {
     const G g = make_g(gd);
+    const Tiles t = rk_tiles(g);
     const double dtdxi = g.dt * g.dx_inv, dtdyi = g.dt * g.dy_inv;  // mod_calcVel.f90:436-437
-    hipLaunchKernelGGL(k_correct_vel, cell_grid(g), dim3(64, 4, 1), 0, s, g, p, u, v, w, dtdxi,
-                       dtdyi);
+    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc));
+    if (mean && fmask)
+        hipLaunchKernelGGL(k_correct<true>, grid, dim3(RK_TX * RK_TY), 0, s, g, t, p, u, v, w,
+                           dtdxi, dtdyi, fmask, vfu, vfv, mean);
+    else
+        hipLaunchKernelGGL(k_correct<false>, grid, dim3(RK_TX * RK_TY), 0, s, g, t, p, u, v, w,
+                           dtdxi, dtdyi, 0, 0.0, 0.0, nullptr);
     return ok();
 }
 

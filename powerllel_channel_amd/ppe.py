@@ -83,6 +83,18 @@ class PoissonSolver:
             raise ValueError("PoissonSolver.solve takes a device tensor")
         check(lib().pll_ppe_solve(self._h, addr, _stream_ptr(stream)))
 
+    def solve_adjoint(self, p, stream=None):
+        """Transposed solve in place (every z system transposed, same gauge
+        pinning); the adjoint of solve()."""
+        if self._h is None:
+            raise RuntimeError("solver was closed")
+        if tuple(p.shape) != self.shape:
+            raise ValueError(f"p has shape {tuple(p.shape)}, expected {self.shape}")
+        addr, dev = _ptr_and_kind(p)
+        if not dev:
+            raise ValueError("PoissonSolver.solve_adjoint takes a device tensor")
+        check(lib().pll_ppe_solve_adjoint(self._h, addr, _stream_ptr(stream)))
+
     def solve_refined(self, p, sweeps: int, stream=None):
         """Refined form: FP32 transforms and solves plus `sweeps` FP64-residual
         correction sweeps (same operator, gauge and in-place semantics)."""

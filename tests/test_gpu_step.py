@@ -138,6 +138,72 @@ def test_one_step_64(orc, cuda):
     assert div < 1e-12
 
 
+def _step_vs_parts(orc, cuda, nx, ny, nz, is_forced):
+    """One fused step and the same step as separate calls, from one start."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields
+    g = Grid(nx, ny, nz, LX, LY, LZ, 3.0)
+    G = orc.OracleGrid(nx, ny, nz, LX, LY, LZ, g.dzf, 1 / 5600, 1e-3)
+    u, v, w = poiseuille_fields(g, 4)
+    orc.bc_vel(G, u, v, w, 0.0)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, 1 / 5600, 1e-3, is_forced=is_forced,
+                 vel_force=(1.0, 0.25, 0.0))
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    z = np.zeros(g.shape)
+    a = _dev(torch, cuda, u, v, w, z, z, z, z)
+    b = _dev(torch, cuda, u, v, w, z, z, z, z)
+    ch.step(s, *a)
+    du, dv, dw, du1, dv1, dw1, dp = b
+    ch.rk1(du, dv, dw, du1, dv1, dw1)
+    ch.bc_vel(du1, dv1, dw1)
+    ch.rk2(du, dv, dw, du1, dv1, dw1)
+    ch.bc_vel(du, dv, dw)
+    ch.calc_rhs(du, dv, dw, dp)
+    s.solve(dp)
+    ch.bc_p(dp)
+    ch.correct_vel(dp, du, dv, dw)
+    ch.force_vel(du, dv, dw)
+    ch.bc_vel(du, dv, dw)
+    torch.cuda.synchronize()
+    return ch, a, b
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(64, 48, 32), (768, 384, 32)])
+def test_fused_rhs_step_bitwise(orc, cuda, nx, ny, nz):
+    """Unforced: the step forms the PPE right-hand side inside the solve's x
+    transform; it must equal calcRHS followed by the solve bit for bit."""
+    import torch
+    print(f"[fused rhs step] {nx}x{ny}x{nz}", flush=True)
+    _, a, b = _step_vs_parts(orc, cuda, nx, ny, nz, (False, False, False))
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("is_forced", [(True, False, False), (True, True, False),
+                                       (False, False, True)])
+def test_fused_forcing_step(orc, cuda, is_forced):
+    """Forced u/v: Mean_h comes from the RK2 output and the shift is applied in
+    correctVel; equal to the separate passes up to summation roundoff, and the
+    forced bulk velocity is met.  A forced w takes the separate passes."""
+    import torch
+    nx, ny, nz = 64, 48, 32
+    print(f"[fused forcing step] forced={is_forced}", flush=True)
+    ch, a, b = _step_vs_parts(orc, cuda, nx, ny, nz, is_forced)
+    for x, y in zip(a[:3], b[:3]):
+        rel = float((x - y).abs().max() / y.abs().max())
+        print(f"  vel rel {rel:.2e}")
+        if is_forced[2]:
+            assert torch.equal(x, y)
+        else:
+            assert rel <= 1e-14
+    for d, target in zip(range(3), (1.0, 0.25, 0.0)):
+        if is_forced[d]:
+            m = ch.mean_h(a[d])
+            assert abs(m - target) <= 1e-13, (d, m)
+
+
 @pytest.mark.slow
 def test_100_steps_192(orc, cuda):
     from powerllel_channel_amd.workloads import CASES
