@@ -24,7 +24,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 HIP_SOURCES = {
-    "fft_kernels.hip": [],
+    # one rounding per transform whatever the I/O policy: no automatic contraction
+    "fft_kernels.hip": ["-ffp-contract=off"],
     "trid_kernels.hip": [],
     "resid_kernels.hip": [],
     # dual-number derivatives: let zero tangents fold away (0 * x = 0)

@@ -210,7 +210,8 @@ static int step_adj(pll_adjoint *a, double *U, hipStream_t s)
     const double *B[3] = {fld(a, a->body, 0), fld(a, a->body, 1), fld(a, a->body, 2)};
     int rc;
     /* stage recomputation: u1 = RK1(u) with its boundary fill */
-    if ((rc = launch(pll_launch_rk1(g, Uc[0], Uc[1], Uc[2], U1[0], U1[1], U1[2], B, s), "rk1")))
+    if ((rc = launch(pll_launch_rk1(g, Uc[0], Uc[1], Uc[2], U1[0], U1[1], U1[2], B, s),
+                     "rk1")))
         return rc;
     if ((rc = pll_channel_bc_vel(ch, U1[0], U1[1], U1[2], s))) return rc;
     /* final boundary fill, forcing, correctVel (velocity part is the identity) */

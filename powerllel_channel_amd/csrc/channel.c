@@ -145,7 +145,8 @@ int pll_channel_rk2(pll_channel *c, double *u, double *v, double *w, const doubl
                     const double *v1, const double *w1, void *stream)
 {
     CHK_C(c);
-    return kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, NULL, (hipStream_t)stream), "rk2");
+    return kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, NULL, (hipStream_t)stream),
+              "rk2");
 }
 
 int pll_channel_calc_rhs(pll_channel *c, const double *u, const double *v, const double *w,
@@ -253,36 +254,36 @@ static int step_impl(pll_channel *c, pll_ppe *ppe, double *u, double *v, double 
      * the summation).  A forced w keeps the reference's separate passes. */
     const int fmask = (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0);
     const int fuse = fmask && !c->is_forced[2];
+    const int none = !fmask && !c->is_forced[2];
     const int nb = pll_rk_blocks(&c->g);
     if ((rc = kl(pll_launch_rk1(&c->g, u, v, w, u1, v1, w1, body, s), "rk1"))) return rc;
     if ((rc = pll_channel_bc_vel(c, u1, v1, w1, stream))) return rc;
-    if (fuse) {
-        if ((rc = kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, c->part, fmask, body, s), "rk2")))
-            return rc;
+    if ((rc = kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, fuse ? c->part : NULL,
+                                fuse ? fmask : 0, body, s),
+                 "rk2")))
+        return rc;
+    if (fuse)
         for (int d = 0; d < 2; d++)
             if ((fmask >> d) & 1)
                 if ((rc = kl(pll_launch_mean_final(nb, c->part + (size_t)d * nb, c->scal + 4 + d,
                                                    c->g.nx, c->g.ny, s),
                              "mean final")))
                     return rc;
-    } else if ((rc = kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, body, s), "rk2"))) {
-        return rc;
-    }
     if ((rc = pll_channel_bc_vel(c, u, v, w, stream))) return rc;
     if (pll_ppe_refinement(ppe) < 0) {
         /* calcRHS fused into the solve's x forward transform: the right-hand
          * side is never stored (mod_calcRHS.f90:28-30, same arithmetic) */
         const double dti = 1.0 / c->g.dt;
         pll_div_src q = {u, v, w, c->g.dzf_inv, dti, dti * c->g.dx_inv, dti * c->g.dy_inv};
-        if ((rc = pll_ppe_solve_div(ppe, &q, p, stream))) return rc;
+        if ((rc = pll_ppe_solve_div(ppe, &q, p, stream))) return rc;  /* + updateBoundP */
     } else {
         if ((rc = pll_channel_calc_rhs(c, u, v, w, p, stream))) return rc;
         if ((rc = pll_ppe_solve(ppe, p, stream))) return rc;
+        if ((rc = pll_channel_bc_p(c, p, stream))) return rc;
     }
-    if ((rc = pll_channel_bc_p(c, p, stream))) return rc;
-    if (fuse) {
-        if ((rc = kl(pll_launch_correct_vel(&c->g, p, u, v, w, fmask, c->vel_force[0],
-                                            c->vel_force[1], c->scal + 4, s),
+    if (fuse || none) {
+        if ((rc = kl(pll_launch_correct_vel(&c->g, p, u, v, w, fuse ? fmask : 0, c->vel_force[0],
+                                            c->vel_force[1], fuse ? c->scal + 4 : NULL, s),
                      "correct_vel")))
             return rc;
     } else {

@@ -54,10 +54,15 @@ template <typename V>
 __device__ __forceinline__ V cscale(V a, typename Sc<V>::T s) { return mk<V>(a.x * s, a.y * s); }
 template <typename V>
 __device__ __forceinline__ V cconj(V a) { return mk<V>(a.x, -a.y); }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// The file is compiled without automatic contraction, so every
+// instantiation of a transform (one per source/destination policy) rounds
+// identically; the complex products use explicit fused multiply-adds.
 template <typename V>
 __device__ __forceinline__ V cmul(V a, V b)
 {
-    return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return mk<V>(fma_(a.x, b.x, -(a.y * b.y)), fma_(a.x, b.y, a.y * b.x));
 }
 // DIR * i * a
 template <int DIR, typename V>
@@ -374,6 +379,38 @@ struct DstHalo64 {  // haloed FP64 array interior := value
         *reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m) = pr;
     }
 };
+struct DstHaloFill {  // haloed FP64 interior := value, plus the boundary fill made from it
+    double *p;
+    long long off1, off2;
+    int nx, ny, nz;
+    // updateBoundP at one rank: periodic images in x and y, zero-gradient
+    // copies of the first/last plane into the ghost planes, corners included.
+    // A row's pair also goes to the row's images (other j ghost row, ghost
+    // planes), and the first/last value of every such row to its x ghosts.
+    __device__ __forceinline__ void row_put(long long rb, int i0, double a, double b) const
+    {
+        d2u pr;
+        pr.x = a;
+        pr.y = b;
+        *reinterpret_cast<d2u *>(p + rb + i0) = pr;
+        if (i0 == 1) p[rb + nx + 1] = a;
+        if (i0 + 1 == nx) p[rb] = b;
+    }
+    __device__ __forceinline__ void store(long long, long long j, long long k, int m, int,
+                                          double a, double b) const
+    {
+        const int i0 = 2 * m + 1, jj = (int)j + 1, kk = (int)k + 1;
+        const int jo = jj == ny ? 0 : (jj == 1 ? ny + 1 : -1);
+        row_put(kk * off1 + jj * off2, i0, a, b);
+        if (jo >= 0) row_put(kk * off1 + jo * off2, i0, a, b);
+        for (int side = 0; side < 2; ++side) {
+            if (side == 0 ? kk != 1 : kk != nz) continue;
+            const long long pg = (side == 0 ? 0 : nz + 1) * off1;
+            row_put(pg + jj * off2, i0, a, b);
+            if (jo >= 0) row_put(pg + jo * off2, i0, a, b);
+        }
+    }
+};
 struct DstDenseSet {  // dense FP64 := value
     double *P;
     __device__ __forceinline__ void store(long long row, long long, long long, int m, int nx,
@@ -498,11 +535,12 @@ k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
     const V *src = W + row * L;
     V v[E];
+    // all row loads in flight before the first LDS write (a load feeding an
+    // LDS write directly is otherwise waited for one at a time)
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int m = t + e * T;
-        ml[m] = valid ? src[m] : mk<V>(S(0), S(0));
-    }
+    for (int e = 0; e < E; ++e) v[e] = valid ? src[t + e * T] : mk<V>(S(0), S(0));
+#pragma unroll
+    for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -788,6 +826,14 @@ extern "C" int pll_launch_xfwd_div(const pll_ppe_dev *d, const pll_div_src *q, v
 extern "C" int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s)
 {
     const DstHalo64 dst{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xb<decltype(f), double2>(d, d->tw_x, W, dst, s);
+    });
+}
+
+extern "C" int pll_launch_xbwd_fill(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s)
+{
+    const DstHaloFill dst{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2, d->nx, d->ny, d->nz};
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xb<decltype(f), double2>(d, d->tw_x, W, dst, s);
     });

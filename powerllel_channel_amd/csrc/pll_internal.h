@@ -96,6 +96,8 @@ int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s);
+/* pll_launch_xbwd that also writes p's boundary fill (updateBoundP). */
+int pll_launch_xbwd_fill(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s);
 /* 1 if the partitioned z solve is instantiated for M rows per lane. */
 int pll_trid_part_supported(int M);
 /* 1 if a kernel instantiation exists for E (elements per thread). */
@@ -186,7 +188,8 @@ int pll_launch_axpy3(const pll_grid_dev *g, double a, const double *const x[3],
 struct pll_ppe;
 /* FP64 solve whose right-hand side is formed from the velocity inside the x
  * forward transform (calcRHS fused into the solve); the solution lands in the
- * interior of d_var as with pll_ppe_solve.  Only for the FP64 form. */
+ * interior of d_var and, unlike pll_ppe_solve, its boundary fill
+ * (updateBoundP) is written too.  Only for the FP64 form. */
 int pll_ppe_solve_div(struct pll_ppe *h, const pll_div_src *q, double *d_var, void *stream);
 /* sweeps of the refined form that pll_ppe_solve uses (-1: FP64 solve). */
 int pll_ppe_refinement(const struct pll_ppe *h);

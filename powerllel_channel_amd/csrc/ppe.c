@@ -479,7 +479,8 @@ static int launch_rc(int rc, const char *what)
 int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream);
 
 /* FP64 solve; the right-hand side is the interior of d_var (q == NULL) or
- * formed from the velocity by the x forward transform. */
+ * formed from the velocity by the x forward transform, in which case the
+ * boundary fill of d_var is written with the solution. */
 static int solve64(pll_ppe *h, const pll_div_src *q, double *d_var, hipStream_t s)
 {
     int rc;
@@ -494,7 +495,8 @@ static int solve64(pll_ppe *h, const pll_div_src *q, double *d_var, hipStream_t 
     if (ev) HIPCHK(hipEventRecord(ev[3], s));
     if ((rc = launch_rc(pll_launch_ybwd(&h->d, h->W, s), "y backward"))) return rc;
     if (ev) HIPCHK(hipEventRecord(ev[4], s));
-    if ((rc = launch_rc(pll_launch_xbwd(&h->d, h->W, d_var, s), "x backward"))) return rc;
+    rc = q ? pll_launch_xbwd_fill(&h->d, h->W, d_var, s) : pll_launch_xbwd(&h->d, h->W, d_var, s);
+    if ((rc = launch_rc(rc, "x backward"))) return rc;
     if (ev) {
         HIPCHK(hipEventRecord(ev[5], s));
         h->nrec++;

@@ -12,6 +12,8 @@
 // Fields are the reference's haloed arrays f(0:nx+1,0:ny+1,0:nz+1), x fastest.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mom_rhs.h"
 #include "pll_internal.h"
 
@@ -51,11 +53,23 @@ __device__ __forceinline__ void rk_tile(const Tiles &t, int &tx, int &ty, int &k
 // MEAN (RK2 only): also the Mean_h partial sums of the new u (fmask bit 0)
 // and v (bit 1), sum of o * dzflzi(k) over the block's cells, into
 // mpart[blockIdx.x] and mpart[nblocks + blockIdx.x].
+//
+// 2.5-D blocking: a workgroup owns a 64 x 4 column tile and marches over a
+// k chunk.  Planes k and k+1 of the three fields sit in LDS as 66 x 6 tiles
+// (tile plus one-cell halo), loaded with coalesced row reads; plane k+2 is
+// fetched into registers while plane k is computed and written over plane
+// k's buffer afterwards.  The few plane k-1 values the stencil reads are
+// carried in registers from the previous step.  Every stencil read is then
+// an LDS read, and each field value is loaded from memory about 1.5 times
+// (the halo) instead of once per neighbour that reads it.
 // BODY: a body force b (haloed layout, interior read) is added to the
 // right-hand side, f := f(q) + b, in both stages.
 struct Body {
     const double *u, *v, *w;
 };
+
+constexpr int LTX = RK_TX + 2, LTY = RK_TY + 2, LPL = LTX * LTY;
+constexpr int NLD = (LPL + RK_TX * RK_TY - 1) / (RK_TX * RK_TY);
 
 template <bool SECOND, bool MEAN, bool BODY>
 __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
@@ -67,29 +81,77 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
                                                      double *__restrict__ mpart, int fmask,
                                                      Body fb)
 {
+    __shared__ double tl[2][3][LPL];
     int tx, ty, kc;
     rk_tile(t, tx, ty, kc);
-    const int i = tx * RK_TX + (int)(threadIdx.x % RK_TX) + 1;
-    const int j = ty * RK_TY + (int)(threadIdx.x / RK_TX) + 1;
+    const int tid = threadIdx.x;
+    const int lx = tid % RK_TX, ly = tid / RK_TX;
+    const int i0 = tx * RK_TX, j0 = ty * RK_TY;  // haloed index of the tile's halo corner
+    const int i = i0 + lx + 1, j = j0 + ly + 1;
+    const bool valid = i <= g.nx && j <= g.ny;
+    const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
+    const long long sy = g.sy, sz = g.sz;
+    const double *__restrict__ Q[3] = {qu, qv, qw};
+    // this thread's share of a plane tile: element q = tid + r * 256
+    long long go[NLD];
+    bool gk[NLD];
+#pragma unroll
+    for (int r = 0; r < NLD; ++r) {
+        const int q = tid + r * RK_TX * RK_TY;
+        const int row = q / LTX, col = q - row * LTX;
+        const int gi = i0 + col, gj = j0 + row;
+        gk[r] = q < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
+        go[r] = gi + sy * gj;
+    }
+    double pre[3][NLD];
+    auto fetch = [&](int kp) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+#pragma unroll
+            for (int r = 0; r < NLD; ++r) pre[f][r] = gk[r] ? Q[f][go[r] + sz * kp] : 0.0;
+    };
+    auto put = [&](int b) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+#pragma unroll
+            for (int r = 0; r < NLD; ++r) {
+                const int q = tid + r * RK_TX * RK_TY;
+                if (q < LPL) tl[b][f][q] = pre[f][r];
+            }
+    };
+    fetch(k0);
+    put(0);
+    fetch(k0 + 1);
+    put(1);
+    long long e = i + sy * j + sz * k0;
+    double u00m = 0.0, v00m = 0.0, w00m = 0.0, wp0m = 0.0, w0pm = 0.0;
+    if (valid) {
+        u00m = qu[e - sz];
+        v00m = qv[e - sz];
+        w00m = qw[e - sz];
+        wp0m = qw[e - sz + 1];
+        w0pm = qw[e - sz + sy];
+    }
+    __syncthreads();
     double su = 0.0, sv = 0.0;
-    if (i <= g.nx && j <= g.ny) {
-        const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
-        const long long sy = g.sy, sz = g.sz;
-        long long e = i + sy * j + sz * k0;
-        Nb n;
-        double u00m = qu[e - sz], v00m = qv[e - sz], w00m = qw[e - sz];
-        double wp0m = qw[e - sz + 1], w0pm = qw[e - sz + sy];
-        double u000 = qu[e], um00 = qu[e - 1], v000 = qv[e], v0m0 = qv[e - sy], w000 = qw[e];
-        for (int k = k0; k <= k1; ++k, e += sz) {
-            n.u000 = u000; n.um00 = um00; n.u00m = u00m;
-            n.up00 = qu[e + 1]; n.u0p0 = qu[e + sy]; n.u0m0 = qu[e - sy];
-            n.ump0 = qu[e - 1 + sy]; n.u00p = qu[e + sz]; n.um0p = qu[e - 1 + sz];
-            n.v000 = v000; n.v0m0 = v0m0; n.v00m = v00m;
-            n.vp00 = qv[e + 1]; n.vm00 = qv[e - 1]; n.v0p0 = qv[e + sy];
-            n.vpm0 = qv[e + 1 - sy]; n.v00p = qv[e + sz]; n.v0mp = qv[e - sy + sz];
-            n.w000 = w000; n.w00m = w00m; n.wp0m = wp0m; n.w0pm = w0pm;
-            n.wp00 = qw[e + 1]; n.wm00 = qw[e - 1]; n.w0p0 = qw[e + sy];
-            n.w0m0 = qw[e - sy]; n.w00p = qw[e + sz];
+    const int c0 = (ly + 1) * LTX + lx + 1;  // this cell in a tile
+    for (int k = k0; k <= k1; ++k, e += sz) {
+        const int b = (k - k0) & 1;
+        const bool more = k + 2 <= k1 + 1;
+        if (more) fetch(k + 2);
+        if (valid) {
+            const double *cu = tl[b][0], *cv = tl[b][1], *cw = tl[b][2];
+            const double *nu = tl[b ^ 1][0], *nv = tl[b ^ 1][1], *nw = tl[b ^ 1][2];
+            Nb n;
+            n.u000 = cu[c0]; n.up00 = cu[c0 + 1]; n.um00 = cu[c0 - 1];
+            n.u0p0 = cu[c0 + LTX]; n.u0m0 = cu[c0 - LTX]; n.ump0 = cu[c0 - 1 + LTX];
+            n.u00p = nu[c0]; n.um0p = nu[c0 - 1]; n.u00m = u00m;
+            n.v000 = cv[c0]; n.vp00 = cv[c0 + 1]; n.vm00 = cv[c0 - 1];
+            n.v0p0 = cv[c0 + LTX]; n.v0m0 = cv[c0 - LTX]; n.vpm0 = cv[c0 + 1 - LTX];
+            n.v00p = nv[c0]; n.v0mp = nv[c0 - LTX]; n.v00m = v00m;
+            n.w000 = cw[c0]; n.wp00 = cw[c0 + 1]; n.wm00 = cw[c0 - 1];
+            n.w0p0 = cw[c0 + LTX]; n.w0m0 = cw[c0 - LTX]; n.w00p = nw[c0];
+            n.w00m = w00m; n.wp0m = wp0m; n.w0pm = w0pm;
             double fu, fv, fw;
             mom_rhs(g, n, k, fu, fv, fw);
             if (BODY) {
@@ -113,25 +175,29 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
                 ov[e] = n.v000 + g.dt * fv;
                 ow[e] = n.w000 + g.dt * fw;
             }
-            u00m = u000; u000 = n.u00p; um00 = n.um0p;
-            v00m = v000; v000 = n.v00p; v0m0 = n.v0mp;
-            w00m = w000; w000 = n.w00p; wp0m = n.wp00; w0pm = n.w0p0;
+            u00m = n.u000;
+            v00m = n.v000;
+            w00m = n.w000;
+            wp0m = n.wp00;
+            w0pm = n.w0p0;
         }
+        __syncthreads();  // plane k's buffer is free
+        if (more) put(b);  // it becomes plane k+2
+        __syncthreads();
     }
     if (MEAN) {
         __shared__ double red[2][RK_TX * RK_TY];
-        const int tt = threadIdx.x;
-        red[0][tt] = su;
-        red[1][tt] = sv;
+        red[0][tid] = su;
+        red[1][tid] = sv;
         __syncthreads();
         for (int h = RK_TX * RK_TY / 2; h > 0; h >>= 1) {
-            if (tt < h) {
-                red[0][tt] += red[0][tt + h];
-                red[1][tt] += red[1][tt + h];
+            if (tid < h) {
+                red[0][tid] += red[0][tid + h];
+                red[1][tid] += red[1][tid + h];
             }
             __syncthreads();
         }
-        if (tt == 0) {
+        if (tid == 0) {
             mpart[blockIdx.x] = red[0][0];
             mpart[gridDim.x + blockIdx.x] = red[1][0];
         }
@@ -416,7 +482,7 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -3; }
 Tiles rk_tiles(const G &g)
 {
     Tiles t;
-    t.kc = 32;
+    t.kc = 64;
     t.ntx = (g.nx + RK_TX - 1) / RK_TX;
     t.nty = (g.ny + RK_TY - 1) / RK_TY;
     t.nkc = (g.nz + t.kc - 1) / t.kc;
@@ -432,6 +498,19 @@ extern "C" int pll_rk_blocks(const pll_grid_dev *gd)
     return t.ntx * t.nty * t.nkc;
 }
 
+// runtime flags -> template instantiation
+template <typename F>
+int with_flags(bool a, bool b, bool c, F &&f)
+{
+#define PLL_B3(A, B, C) \
+    if (a == A && b == B && c == C) return f(std::integral_constant<bool, A>{}, std::integral_constant<bool, B>{}, std::integral_constant<bool, C>{});
+    PLL_B3(false, false, false) PLL_B3(false, false, true) PLL_B3(false, true, false)
+    PLL_B3(false, true, true) PLL_B3(true, false, false) PLL_B3(true, false, true)
+    PLL_B3(true, true, false) PLL_B3(true, true, true)
+#undef PLL_B3
+    return -1;
+}
+
 extern "C" int pll_launch_rk1(const pll_grid_dev *gd, const double *u, const double *v,
                               const double *w, double *u1, double *v1, double *w1,
                               const double *const *body, hipStream_t s)
@@ -439,39 +518,30 @@ extern "C" int pll_launch_rk1(const pll_grid_dev *gd, const double *u, const dou
     const G g = make_g(gd);
     const Tiles t = rk_tiles(g);
     const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
-    if (body) {
-        const Body fb{body[0], body[1], body[2]};
-        hipLaunchKernelGGL((k_rk<false, false, true>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
-                           w1, nullptr, 0, fb);
-    } else {
-        hipLaunchKernelGGL((k_rk<false, false, false>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
-                           w1, nullptr, 0, Body{nullptr, nullptr, nullptr});
-    }
-    return ok();
+    const Body fb = body ? Body{body[0], body[1], body[2]} : Body{nullptr, nullptr, nullptr};
+    return with_flags(false, body != nullptr, false, [&](auto, auto B, auto) {
+        hipLaunchKernelGGL((k_rk<false, false, decltype(B)::value>), grid, blk, 0, s, g, t, u, v,
+                           w, u1, v1, w1, nullptr, 0, fb);
+        return ok();
+    });
 }
 
 extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, double *w,
                               const double *u1, const double *v1, const double *w1,
-                              double *mpart, int fmask, const double *const *body, hipStream_t s)
+                              double *mpart, int fmask, const double *const *body,
+                              hipStream_t s)
 {
     const G g = make_g(gd);
     const Tiles t = rk_tiles(g);
     const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
-    const bool mean = mpart && fmask;
     const Body fb = body ? Body{body[0], body[1], body[2]} : Body{nullptr, nullptr, nullptr};
-    if (mean && body)
-        hipLaunchKernelGGL((k_rk<true, true, true>), grid, blk, 0, s, g, t, u1, v1, w1, u, v, w,
-                           mpart, fmask, fb);
-    else if (mean)
-        hipLaunchKernelGGL((k_rk<true, true, false>), grid, blk, 0, s, g, t, u1, v1, w1, u, v, w,
-                           mpart, fmask, fb);
-    else if (body)
-        hipLaunchKernelGGL((k_rk<true, false, true>), grid, blk, 0, s, g, t, u1, v1, w1, u, v, w,
-                           nullptr, 0, fb);
-    else
-        hipLaunchKernelGGL((k_rk<true, false, false>), grid, blk, 0, s, g, t, u1, v1, w1, u, v,
-                           w, nullptr, 0, fb);
-    return ok();
+    const bool mean = mpart && fmask;
+    return with_flags(mean, body != nullptr, false, [&](auto M, auto B, auto) {
+        hipLaunchKernelGGL((k_rk<true, decltype(M)::value, decltype(B)::value>), grid, blk, 0, s,
+                           g, t, u1, v1, w1, u, v, w, mean ? mpart : nullptr, mean ? fmask : 0,
+                           fb);
+        return ok();
+    });
 }
 
 extern "C" int pll_launch_calc_rhs(const pll_grid_dev *gd, const double *u, const double *v,
@@ -493,13 +563,13 @@ extern "C" int pll_launch_correct_vel(const pll_grid_dev *gd, const double *p, d
     const Tiles t = rk_tiles(g);
     const double dtdxi = g.dt * g.dx_inv, dtdyi = g.dt * g.dy_inv;  // mod_calcVel.f90:436-437
     const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc));
-    if (mean && fmask)
-        hipLaunchKernelGGL(k_correct<true>, grid, dim3(RK_TX * RK_TY), 0, s, g, t, p, u, v, w,
-                           dtdxi, dtdyi, fmask, vfu, vfv, mean);
-    else
-        hipLaunchKernelGGL(k_correct<false>, grid, dim3(RK_TX * RK_TY), 0, s, g, t, p, u, v, w,
-                           dtdxi, dtdyi, 0, 0.0, 0.0, nullptr);
-    return ok();
+    const bool force = mean && fmask;
+    return with_flags(force, false, false, [&](auto Fo, auto, auto) {
+        hipLaunchKernelGGL(k_correct<decltype(Fo)::value>, grid, dim3(RK_TX * RK_TY), 0, s, g, t, p,
+                           u, v, w, dtdxi, dtdyi, force ? fmask : 0, vfu, vfv,
+                           force ? mean : nullptr);
+        return ok();
+    });
 }
 
 extern "C" int pll_launch_mean_partials(const pll_grid_dev *gd, const double *q, double *part,
