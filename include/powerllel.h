@@ -160,6 +160,17 @@ int pll_channel_mean_h(pll_channel *c, const double *q, double *mean,
 int pll_channel_max_div(pll_channel *c, const double *u, const double *v,
                         const double *w, double *value, int ijk[3],
                         void *stream);
+/* calcMaxCFL, src/mod_utils.f90:193-255: max over the interior of
+ * dt (|u_i + u_{i-1}|/2 dx_inv + |v_j + v_{j-1}|/2 dy_inv + |w_k + w_{k-1}|/2 dzf_inv(k))
+ * and its 1-based (i,j,k) (first in k,j,i order among equal maxima; NaN
+ * cells are skipped, as by the reference's comparisons).  Synchronises. */
+int pll_channel_max_cfl(pll_channel *c, const double *u, const double *v,
+                        const double *w, double *value, int ijk[3],
+                        void *stream);
+/* checkNaN, src/mod_utils.f90:352-394: *has_nan = 1 if any element of the
+ * haloed field is NaN.  Synchronises. */
+int pll_channel_check_nan(pll_channel *c, const double *q, int *has_nan,
+                          void *stream);
 /* One whole time step, src/main.f90:137-208: RK1, BC, RK2, BC, calcRHS,
  * PPE (ppe must match the grid), updateBoundP, correctVel, forceVel, BC.
  * With the FP64 solve, calcRHS is formed inside the solve's x transform

@@ -903,6 +903,30 @@ double orc_max_div(const orc_grid_t *g, const double *u, const double *v,
     return best;
 }
 
+double orc_max_cfl(const orc_grid_t *g, const double *u, const double *v,
+                   const double *w, int ijk[3])
+{ /* mod_utils.f90:193-255: first location (k,j,i order) of the maximum, times dt */
+    double best = 0.0;
+    int bi = 1, bj = 1, bk = 1;
+    for (int k = 1; k <= g->nz; k++)
+        for (int j = 1; j <= g->ny; j++)
+            for (int i = 1; i <= g->nx; i++) {
+                double c = fabs((u[IX(g, i, j, k)] + u[IX(g, i - 1, j, k)]) * 0.5) * g->dx_inv +
+                           fabs((v[IX(g, i, j, k)] + v[IX(g, i, j - 1, k)]) * 0.5) * g->dy_inv +
+                           fabs((w[IX(g, i, j, k)] + w[IX(g, i, j, k - 1)]) * 0.5) * g->dzf_inv[k];
+                if (c > best) { best = c; bi = i; bj = j; bk = k; }
+            }
+    if (ijk) { ijk[0] = bi; ijk[1] = bj; ijk[2] = bk; }
+    return best * g->dt;
+}
+
+int orc_has_nan(long n, const double *q)
+{ /* mod_utils.f90:352-394 over the whole array */
+    for (long t = 0; t < n; t++)
+        if (q[t] != q[t]) return 1;
+    return 0;
+}
+
 double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf)
 { /* mod_monitor.f90:225-258 at both walls (k = 1 and k = nz), averaged */
     double tau[2];

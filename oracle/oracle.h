@@ -101,6 +101,9 @@ void orc_bc_p(const orc_grid_t *g, double *p);
 double orc_max_div(const orc_grid_t *g, const double *u, const double *v,
                    const double *w, int ijk[3]);
 double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf);
+double orc_max_cfl(const orc_grid_t *g, const double *u, const double *v,
+                   const double *w, int ijk[3]);
+int orc_has_nan(long n, const double *q);
 /* orc_step with a body force (bu, bv, bw) added to the momentum right-hand
  * side of both RK stages (for the adjoint checks; not in the reference) */
 void orc_step_body(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,

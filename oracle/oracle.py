@@ -73,6 +73,8 @@ def lib():
         "orc_bc_p": (None, [gp, _dp]),
         "orc_max_div": (C.c_double, [gp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
         "orc_wall_shear": (C.c_double, [gp, _dp, C.c_double]),
+        "orc_max_cfl": (C.c_double, [gp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
+        "orc_has_nan": (C.c_int, [C.c_long, _dp]),
         "orc_step": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                             C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "orc_step_body": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
@@ -234,6 +236,16 @@ def max_div(G, u, v, w):
     ijk = (C.c_int * 3)()
     val = lib().orc_max_div(G.ref(), u, v, w, ijk)
     return val, (ijk[0], ijk[1], ijk[2])
+
+
+def max_cfl(G, u, v, w):
+    ijk = (C.c_int * 3)()
+    val = lib().orc_max_cfl(G.ref(), u, v, w, ijk)
+    return val, (ijk[0], ijk[1], ijk[2])
+
+
+def has_nan(q):
+    return bool(lib().orc_has_nan(q.size, q))
 
 
 def wall_shear(G, u, u_crf=0.0):

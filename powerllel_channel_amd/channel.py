@@ -93,6 +93,34 @@ class Channel:
                                         _stream_ptr(stream)))
         return val.value, (ijk[0], ijk[1], ijk[2])
 
+    def max_cfl(self, u, v, w, stream=None):
+        """calcMaxCFL: (value * dt, (i, j, k) 1-based)."""
+        self._chk(u, v, w)
+        val = C.c_double()
+        ijk = (C.c_int * 3)()
+        check(lib().pll_channel_max_cfl(self._h, _p(u), _p(v), _p(w), C.byref(val), ijk,
+                                        _stream_ptr(stream)))
+        return val.value, (ijk[0], ijk[1], ijk[2])
+
+    def check_nan(self, q, stream=None) -> bool:
+        """checkNaN over every element of the haloed field: True if a NaN is present."""
+        self._chk(q)
+        out = C.c_int()
+        check(lib().pll_channel_check_nan(self._h, _p(q), C.byref(out), _stream_ptr(stream)))
+        return bool(out.value)
+
+    def check(self, u, v, w, cfl_limit, div_limit, stream=None):
+        """The run-time check of src/main.f90:213-222: checkNaN(u), calcMaxCFL +
+        checkCFL, calcMaxDiv + checkDiv.  Returns (passed, report); a value
+        above its limit or NaN fails, as in checkCFL / checkDiv."""
+        nan = self.check_nan(u, stream)
+        cfl, cfl_at = self.max_cfl(u, v, w, stream)
+        div, div_at = self.max_div(u, v, w, stream)
+        ok_cfl = not (cfl > cfl_limit or cfl != cfl)
+        ok_div = not (div > div_limit or div != div)
+        report = {"nan": nan, "cfl_max": cfl, "cfl_at": cfl_at, "div_max": div, "div_at": div_at}
+        return (not nan) and ok_cfl and ok_div, report
+
     def step(self, ppe: PoissonSolver, u, v, w, u1, v1, w1, p, stream=None):
         self._chk(u, v, w, u1, v1, w1, p)
         check(lib().pll_channel_step(self._h, ppe.handle, _p(u), _p(v), _p(w), _p(u1), _p(v1),

@@ -239,6 +239,55 @@ int pll_channel_max_div(pll_channel *c, const double *u, const double *v, const 
     return PLL_OK;
 }
 
+/* calcMaxCFL, mod_utils.f90:193-255: max over the interior of the CFL sum
+ * times dt, and its 1-based (i,j,k). */
+int pll_channel_max_cfl(pll_channel *c, const double *u, const double *v, const double *w,
+                        double *value, int ijk[3], void *stream)
+{
+    CHK_C(c);
+    if (!u || !v || !w) return pll_set_error(PLL_EINVAL, "NULL field");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if ((rc = kl(pll_launch_maxcfl_partials(&c->g, u, v, w, c->part, c->pidx, s), "maxcfl")))
+        return rc;
+    if ((rc = kl(pll_launch_maxdiv_final(c->g.ny * c->g.nz, c->part, c->pidx, c->scal + 1,
+                                         c->sidx, s),
+                 "maxcfl final")))
+        return rc;
+    double val;
+    long long idx;
+    HIPCHK(hipMemcpyAsync(&val, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&idx, c->sidx, sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (value) *value = (val < 0.0 ? 0.0 : val) * c->g.dt;
+    if (ijk) {
+        const long long nx = c->g.nx, ny = c->g.ny;
+        ijk[0] = (int)(idx % nx) + 1;
+        ijk[1] = (int)((idx / nx) % ny) + 1;
+        ijk[2] = (int)(idx / (nx * ny)) + 1;
+    }
+    return PLL_OK;
+}
+
+/* checkNaN, mod_utils.f90:352-394, over every element of the haloed field. */
+int pll_channel_check_nan(pll_channel *c, const double *q, int *has_nan, void *stream)
+{
+    CHK_C(c);
+    if (!q || !has_nan) return pll_set_error(PLL_EINVAL, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    int nb = c->g.ny * c->g.nz;
+    if (nb > 1024) nb = 1024;
+    int rc;
+    if ((rc = kl(pll_launch_nan_partials(&c->g, q, c->pidx, nb, s), "checkNaN"))) return rc;
+    long long flags[1024];
+    HIPCHK(hipMemcpyAsync(flags, c->pidx, sizeof(long long) * (size_t)nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int any = 0;
+    for (int b = 0; b < nb; b++) any |= flags[b] != 0;
+    *has_nan = any;
+    return PLL_OK;
+}
+
 static int step_impl(pll_channel *c, pll_ppe *ppe, double *u, double *v, double *w, double *u1,
                      double *v1, double *w1, double *p, const double *const *body, void *stream)
 {

@@ -455,6 +455,16 @@ struct XArgs {
     double norm;
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin; renumbering them so
+// that each XCD runs one contiguous range of tiles puts neighbouring column
+// tiles (which share 128-byte lines when a tile row is narrower than a
+// line) on one L2.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned b = blockIdx.x, n = gridDim.x;
+    return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 template <typename V>
 struct YArgs {
     pll_fftplan P;
@@ -577,8 +587,9 @@ k_yfwd(V *__restrict__ W, YArgs<V> a)
     V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
-    const int tile = blockIdx.x % a.ntiles;
-    const long long k = blockIdx.x / a.ntiles;
+    const unsigned b = xcd_block();
+    const int tile = b % a.ntiles;
+    const long long k = b / a.ntiles;
     const int c = threadIdx.x % a.cols, t = threadIdx.x / a.cols;
     const int mx = tile * a.cols + c;
     const bool valid = mx < L;
@@ -627,8 +638,9 @@ k_ybwd(V *__restrict__ W, YArgs<V> a)
     V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
-    const int tile = blockIdx.x % a.ntiles;
-    const long long k = blockIdx.x / a.ntiles;
+    const unsigned b = xcd_block();
+    const int tile = b % a.ntiles;
+    const long long k = b / a.ntiles;
     const int c = threadIdx.x % a.cols, t = threadIdx.x / a.cols;
     const int mx = tile * a.cols + c;
     const bool valid = mx < L;

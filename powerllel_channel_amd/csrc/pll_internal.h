@@ -151,6 +151,13 @@ int pll_launch_maxdiv_partials(const pll_grid_dev *g, const double *u,
                                long long *pidx, hipStream_t s);
 int pll_launch_maxdiv_final(int n, const double *pval, const long long *pidx,
                             double *oval, long long *oidx, hipStream_t s);
+/* the same reduction for the CFL sum (calcMaxCFL before the factor dt) */
+int pll_launch_maxcfl_partials(const pll_grid_dev *g, const double *u,
+                               const double *v, const double *w, double *pval,
+                               long long *pidx, hipStream_t s);
+/* flags[0 .. nblocks) = 1 where a block's share of the haloed field holds a NaN */
+int pll_launch_nan_partials(const pll_grid_dev *g, const double *q, long long *flags,
+                            int nblocks, hipStream_t s);
 
 /* ---- adjoint pieces (adjoint_kernels.hip); every array is haloed ---- */
 /* o = beta o + a0 l [interior] + alpha J_f(q)^T l, over all cells; l read on

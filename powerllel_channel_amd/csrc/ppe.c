@@ -216,14 +216,17 @@ static int choose_part(int nx, int nz, int *M, int *P, int *S)
 {
     static const int Ms[] = {8, 12, 16, 4};
     static const int Ss[] = {16, 8, 4};
+    const char *pm = getenv("PLL_TRID_M"); /* tuning override of the rows per lane */
+    const int want = pm ? atoi(pm) : 0;
     for (size_t i = 0; i < sizeof Ms / sizeof Ms[0]; i++) {
         const int m = Ms[i];
+        if (want && m != want) continue;
         if (nz % m || !pll_trid_part_supported(m)) continue;
         const int p = nz / m;
         if (p < 2 || p > 64 || (p & (p - 1))) continue;
         for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
             const int sc = Ss[j];
-            if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 > 163840) continue;
+            if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 + (size_t)nz * 24 > 163840) continue;
             *M = m;
             *P = p;
             *S = sc;
@@ -379,6 +382,21 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
                     const int ex = (q0 < qx && iy < iyx);
                     if (ex == (pass == 0)) order[n++] = t;
                 }
+            /* workgroup b runs tile order[b]; workgroups reach the 8 XCDs
+             * round-robin, so hand each XCD a contiguous run of this list:
+             * neighbouring tiles (sharing 128-byte lines in FP32 storage)
+             * then meet in one L2 */
+            if (ntiles % 8 == 0) {
+                int *perm = (int *)malloc(sizeof(int) * (size_t)ntiles);
+                if (!perm) {
+                    free(order);
+                    rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
+                    goto fail;
+                }
+                for (int b = 0; b < ntiles; b++) perm[b] = order[(b % 8) * (ntiles / 8) + b / 8];
+                free(order);
+                order = perm;
+            }
             double *lxh = (double *)malloc(sizeof(double) * (size_t)(L + 1));
             if (!lxh) {
                 free(order);

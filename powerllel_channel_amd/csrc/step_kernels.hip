@@ -398,7 +398,13 @@ __global__ void __launch_bounds__(256) k_bc_p(G g, double *__restrict__ p, long 
 
 // One workgroup per interior (j,k) row: max |div| and the smallest linear
 // index (k-1)*nx*ny + (j-1)*nx + (i-1) attaining it.
-__global__ void __launch_bounds__(256) k_maxdiv_part(G g, const double *__restrict__ u,
+// KIND 0: |div u| (calcMaxDiv, mod_utils.f90:286-288);
+// KIND 1: the CFL sum before the factor dt (calcMaxCFL, mod_utils.f90:224-226)
+//   |(u_i + u_{i-1})/2| dx_inv + |(v_j + v_{j-1})/2| dy_inv + |(w_k + w_{k-1})/2| dzf_inv(k).
+// NaN cells never win (every comparison with NaN is false), as in the
+// reference's `if (x > max)` loops.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_maxloc_part(G g, const double *__restrict__ u,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ w,
                                                       double *__restrict__ pval,
@@ -412,8 +418,13 @@ __global__ void __launch_bounds__(256) k_maxdiv_part(G g, const double *__restri
     long long bi = 0;
     for (int i = threadIdx.x + 1; i <= g.nx; i += blockDim.x) {
         const long long e = base + i;
-        const double d = fabs((u[e] - u[e - 1]) * g.dx_inv + (v[e] - v[e - g.sy]) * g.dy_inv +
-                              (w[e] - w[e - g.sz]) * g.dzf_inv[k]);
+        double d;
+        if (KIND == 0)
+            d = fabs((u[e] - u[e - 1]) * g.dx_inv + (v[e] - v[e - g.sy]) * g.dy_inv +
+                     (w[e] - w[e - g.sz]) * g.dzf_inv[k]);
+        else
+            d = fabs((u[e] + u[e - 1]) * 0.5) * g.dx_inv + fabs((v[e] + v[e - g.sy]) * 0.5) * g.dy_inv +
+                fabs((w[e] + w[e - g.sz]) * 0.5) * g.dzf_inv[k];
         if (d > best) {
             best = d;
             bi = (long long)(k - 1) * g.nx * g.ny + (long long)(j - 1) * g.nx + (i - 1);
@@ -477,6 +488,19 @@ __global__ void __launch_bounds__(1024) k_maxdiv_final(int n, const double *__re
 }
 
 inline dim3 cell_grid(const G &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
+// checkNaN (mod_utils.f90:352-394) over every element of a haloed field:
+// flags[block] = 1 if the block's stride of elements holds a NaN.
+__global__ void __launch_bounds__(256) k_nan_part(const double *__restrict__ q, long long n,
+                                                  long long *__restrict__ flags)
+{
+    int bad = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        bad |= q[t] != q[t];
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) flags[blockIdx.x] = bad;
+}
+
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -3; }
 
 Tiles rk_tiles(const G &g)
@@ -621,7 +645,25 @@ extern "C" int pll_launch_maxdiv_partials(const pll_grid_dev *gd, const double *
                                           long long *pidx, hipStream_t s)
 {
     const G g = make_g(gd);
-    hipLaunchKernelGGL(k_maxdiv_part, dim3(g.ny * g.nz), dim3(256), 0, s, g, u, v, w, pval, pidx);
+    hipLaunchKernelGGL(k_maxloc_part<0>, dim3(g.ny * g.nz), dim3(256), 0, s, g, u, v, w, pval, pidx);
+    return ok();
+}
+
+extern "C" int pll_launch_maxcfl_partials(const pll_grid_dev *gd, const double *u, const double *v,
+                                          const double *w, double *pval, long long *pidx,
+                                          hipStream_t s)
+{
+    const G g = make_g(gd);
+    hipLaunchKernelGGL(k_maxloc_part<1>, dim3(g.ny * g.nz), dim3(256), 0, s, g, u, v, w, pval, pidx);
+    return ok();
+}
+
+extern "C" int pll_launch_nan_partials(const pll_grid_dev *gd, const double *q, long long *flags,
+                                       int nblocks, hipStream_t s)
+{
+    const G g = make_g(gd);
+    const long long n = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    hipLaunchKernelGGL(k_nan_part, dim3(nblocks), dim3(256), 0, s, q, n, flags);
     return ok();
 }
 

@@ -163,12 +163,22 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
     const long long plane = (long long)g.nx * g.ny;
     St *__restrict__ base = Wr + (long long)my * g.nx + q0;
     const int nthr = S * P;
+    // z coefficients staged in LDS after the tile and its pivot half, as
+    // [a | c | bz] each in [i][p] order (row k = p*M + i at i*P + p), so the
+    // P lanes of a column read consecutive words
+    double *cf = tl + 2 * (size_t)nz * S;
 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         const int e = tid + j * nthr;
         const int k = e / S, s = e - k * S;
         tl[swz(k, s, S, M)] = (double)base[k * plane + s];
+    }
+    for (int k = tid; k < nz; k += nthr) {
+        const int o = (k % M) * P + k / M;
+        cf[o] = g.a[k];
+        cf[nz + o] = g.c[k];
+        cf[2 * nz + o] = g.bz[k];
     }
     __syncthreads();
 
@@ -191,7 +201,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 #pragma unroll 4
             for (int k = 1; k < nz; ++k) {
                 const int o = swz(k, s, S, M);
-                const double m = g.a[k] / pt[swz(k - 1, s, S, M)];
+                const double m = cf[(k % M) * P + k / M] / pt[swz(k - 1, s, S, M)];
                 const double cur = tl[o] - m * prev;
                 tl[o] = cur;
                 prev = cur;
@@ -203,7 +213,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 #pragma unroll 4
             for (int k = nz - 2; k >= 0; --k) {
                 const int o = swz(k, s, S, M);
-                const double cur = (tl[o] - g.c[k] * prev) / pt[o];
+                const double cur = (tl[o] - cf[nz + (k % M) * P + k / M] * prev) / pt[o];
                 tl[o] = cur;
                 prev = cur;
             }
@@ -217,21 +227,27 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 #pragma unroll
         for (int i = 0; i < M; ++i) d[i] = tl[swz(k0 + i, s, S, M)];
         // 1. local elimination
+#define CA(i) cf[(i) * P + p]
+#define CC(i) cf[nz + (i) * P + p]
+#define CB(i) cf[2 * nz + (i) * P + p]
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const double r = rcp(g.bz[k0 + i] + lam);
+            const double r = rcp(CB(i) + lam);
             d[i] *= r;
-            ap[i] = g.a[k0 + i] * r;
-            cp[i] = g.c[k0 + i] * r;
+            ap[i] = CA(i) * r;
+            cp[i] = CC(i) * r;
         }
 #pragma unroll
         for (int i = 2; i < M; ++i) {
-            const double ai = g.a[k0 + i];
-            const double r = rcp((g.bz[k0 + i] + lam) - ai * cp[i - 1]);
+            const double ai = CA(i);
+            const double r = rcp((CB(i) + lam) - ai * cp[i - 1]);
             d[i] = r * (d[i] - ai * d[i - 1]);
             ap[i] = -r * ai * ap[i - 1];
-            cp[i] = r * g.c[k0 + i];
+            cp[i] = r * CC(i);
         }
+#undef CA
+#undef CC
+#undef CB
 #pragma unroll
         for (int i = M - 3; i >= 1; --i) {
             d[i] -= cp[i] * d[i + 1];
@@ -334,7 +350,8 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         g.iyx = d->tp.iyx;
         const unsigned blocks = (unsigned)(g.ntx * d->ny);
         const int threads = g.S * g.P;
-        const size_t shm = 2 * sizeof(double) * (size_t)d->nz * g.S;  // values + pivots
+        // values + pivots + coefficients
+        const size_t shm = sizeof(double) * (2 * (size_t)d->nz * g.S + 3 * (size_t)d->nz);
         if (threads > 512 || shm > 163840) return -1;
         switch (d->tp.M) {
 #define PLL_TRID_CASE(M_)                                                                       \

@@ -216,3 +216,31 @@ def test_100_steps_192(orc, cuda):
         assert rel <= 1e-10
     print(f"  max div {div:.2e}")
     assert div < 1e-12
+
+
+def test_monitors_vs_oracle(orc, cuda):
+    """calcMaxCFL / calcMaxDiv / checkNaN on the GPU against the oracle, and the
+    run-time check of main.f90:213-222 (NaN planted in the interior and in a
+    halo cell)."""
+    import torch
+    g, G, ch, f = _setup(orc, 48, 32, 24, 3.0)
+    u, v, w = f[0], f[1], f[2]
+    du, dv, dw = _dev(torch, cuda, u, v, w)
+    val, ijk = ch.max_cfl(du, dv, dw)
+    oval, oijk = orc.max_cfl(G, u, v, w)
+    print(f"[monitors] cfl gpu {val!r} {ijk} oracle {oval!r} {oijk}")
+    assert val == oval and ijk == oijk
+    dval, dijk = ch.max_div(du, dv, dw)
+    odval, odijk = orc.max_div(G, u, v, w)
+    assert dval == odval and dijk == odijk
+    assert not ch.check_nan(du)
+    passed, rep = ch.check(du, dv, dw, cfl_limit=1e9, div_limit=1e9)
+    assert passed and not rep["nan"]
+    passed, rep = ch.check(du, dv, dw, cfl_limit=0.5 * val, div_limit=1e9)
+    assert not passed
+    for idx in [(5, 7, 9), (0, 0, 0), (g.nz + 1, g.ny + 1, g.nx + 1)]:
+        bad = du.clone()
+        bad[idx] = float("nan")
+        assert ch.check_nan(bad)
+        passed, rep = ch.check(bad, dv, dw, cfl_limit=1e9, div_limit=1e9)
+        assert not passed and rep["nan"]

@@ -156,3 +156,30 @@ def test_wall_shear_laminar(orc):
         u = np.broadcast_to((1.5 * (1 - zc ** 2))[:, None, None], G.shape).copy()
         errs.append(abs(orc.wall_shear(G, u) - 3 * 0.01))
     assert errs[2] < errs[1] < errs[0] and errs[2] < 2e-3
+
+
+def test_max_cfl_uniform_and_planted(orc):
+    """calcMaxCFL: a uniform flow (U, V, W) gives dt (|U|/dx + |V|/dy + |W|/dzf_k)
+    at the first plane of smallest dzf; a planted u spike at (i0, j0, k0)
+    is seen by cells i0 and i0+1, the first of which wins a tie."""
+    G = _grid(orc, 8, 6, 7, 3.0, dt=2e-3)
+    U, V, W = 0.7, -0.3, 0.2
+    u = np.full(G.shape, U)
+    v = np.full(G.shape, V)
+    w = np.full(G.shape, W)
+    val, ijk = orc.max_cfl(G, u, v, w)
+    dzf = G.dzf[1:-1]
+    kmin = int(np.argmin(dzf)) + 1
+    expect = 2e-3 * (abs(U) / G.dx + abs(V) / G.dy + abs(W) / dzf.min())
+    assert abs(val - expect) <= 1e-14 * expect and ijk == (1, 1, kmin)
+    u = np.zeros(G.shape)
+    u[4, 3, 5] = 10.0  # k=4, j=3, i=5
+    val, ijk = orc.max_cfl(G, u, np.zeros(G.shape), np.zeros(G.shape))
+    assert ijk == (5, 3, 4) and abs(val - 2e-3 * 5.0 / G.dx) < 1e-12
+
+
+def test_has_nan(orc):
+    q = np.zeros((5, 6, 7))
+    assert not orc.has_nan(q)
+    q[0, 0, 6] = np.nan  # a ghost corner
+    assert orc.has_nan(q)
