@@ -50,6 +50,7 @@ typedef struct {
     const double *piv;         /* [nz][ny/2+1][L+1] factorised diagonal */
     struct {                   /* partitioned z solve (P = 0: streaming solve) */
         int M, P, S;           /* rows per lane, lanes per column, columns per tile */
+        int dpp16;             /* P == 16: interface reduction by DPP row shifts */
         int qx, iyx;           /* tiles with q0 < qx and iy < iyx: exact path */
         const double *bz;      /* [nz] diagonal before adding the eigenvalue */
         const double *lamx;    /* [L+1]    */

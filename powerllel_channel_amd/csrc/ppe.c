@@ -212,27 +212,33 @@ static int choose_launch(pll_ppe *h)
  * one wavefront), S columns per tile (16, 8 or 4 doubles = one 128/64/32-byte
  * row segment); LDS holds the tile twice (values and, for low-mode tiles,
  * pivots).  Returns 0 (streaming solve) if nothing fits. */
+static int part_fits(int nx, int nz, int m, int *M, int *P, int *S)
+{
+    static const int Ss[] = {16, 8, 4};
+    if (m <= 0 || nz % m || !pll_trid_part_supported(m)) return 0;
+    const int p = nz / m;
+    if (p < 2 || p > 64 || (p & (p - 1))) return 0;
+    for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
+        const int sc = Ss[j];
+        if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 + (size_t)nz * 24 > 163840) continue;
+        *M = m;
+        *P = p;
+        *S = sc;
+        return 1;
+    }
+    return 0;
+}
+
 static int choose_part(int nx, int nz, int *M, int *P, int *S)
 {
     static const int Ms[] = {8, 12, 16, 4};
-    static const int Ss[] = {16, 8, 4};
     const char *pm = getenv("PLL_TRID_M"); /* tuning override of the rows per lane */
-    const int want = pm ? atoi(pm) : 0;
-    for (size_t i = 0; i < sizeof Ms / sizeof Ms[0]; i++) {
-        const int m = Ms[i];
-        if (want && m != want) continue;
-        if (nz % m || !pll_trid_part_supported(m)) continue;
-        const int p = nz / m;
-        if (p < 2 || p > 64 || (p & (p - 1))) continue;
-        for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
-            const int sc = Ss[j];
-            if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 + (size_t)nz * 24 > 163840) continue;
-            *M = m;
-            *P = p;
-            *S = sc;
-            return 1;
-        }
-    }
+    if (pm) return part_fits(nx, nz, atoi(pm), M, P, S);
+    /* prefer P = 16 lanes per column (one DPP row) unless PLL_TRID_DPP=0 */
+    const char *pd = getenv("PLL_TRID_DPP");
+    if (!(pd && atoi(pd) == 0) && nz % 16 == 0 && part_fits(nx, nz, nz / 16, M, P, S)) return 1;
+    for (size_t i = 0; i < sizeof Ms / sizeof Ms[0]; i++)
+        if (part_fits(nx, nz, Ms[i], M, P, S)) return 1;
     return 0;
 }
 
@@ -420,6 +426,10 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
             h->d.tp.M = M;
             h->d.tp.P = P;
             h->d.tp.S = S;
+            {
+                const char *pd = getenv("PLL_TRID_DPP");
+                h->d.tp.dpp16 = (P == 16) && !(pd && atoi(pd) == 0);
+            }
             h->d.tp.qx = qx;
             h->d.tp.iyx = iyx;
             h->d.tp.bz = (const double *)h->bz;

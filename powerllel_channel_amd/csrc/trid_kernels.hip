@@ -131,6 +131,38 @@ __device__ __forceinline__ double rcp(double x)
     return __builtin_fma(r, e, r);
 }
 
+// Lane exchange within 16-lane rows (DPP): ROW_SHR(n) gives lane l the value
+// of lane l-n, ROW_SHL(n) that of lane l+n, and 0 when that lane lies outside
+// l's row (the `old` operand is 0 and bound_ctrl is set).  With the 16 lanes
+// of a column forming one row, a missing neighbour is read as the empty
+// equation without any select.
+constexpr int ROW_SHL(int n) { return 0x100 + n; }
+constexpr int ROW_SHR(int n) { return 0x110 + n; }
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+template <int T>
+__device__ __forceinline__ void pcr16_stage(double &fa, double &fg, double &fd, double &la,
+                                            double &lg, double &ld)
+{
+    const double pfa = dpp<ROW_SHR(T)>(fa), pfg = dpp<ROW_SHR(T)>(fg), pfd = dpp<ROW_SHR(T)>(fd);
+    const double nfa = dpp<ROW_SHL(T)>(fa), nfg = dpp<ROW_SHL(T)>(fg), nfd = dpp<ROW_SHL(T)>(fd);
+    const double pla = dpp<ROW_SHR(T)>(la), plg = dpp<ROW_SHR(T)>(lg), pld = dpp<ROW_SHR(T)>(ld);
+    const double nla = dpp<ROW_SHL(T)>(la), nlg = dpp<ROW_SHL(T)>(lg), nld = dpp<ROW_SHL(T)>(ld);
+    const double rf = rcp(1.0 - fa * pfg - fg * nfa);
+    const double fa2 = -fa * pfa * rf, fg2 = -fg * nfg * rf;
+    const double fd2 = (fd - fa * pfd - fg * nfd) * rf;
+    const double rl = rcp(1.0 - la * plg - lg * nla);
+    const double la2 = -la * pla * rl, lg2 = -lg * nlg * rl;
+    const double ld2 = (ld - la * pld - lg * nld) * rl;
+    fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
+}
+
 __device__ __forceinline__ int swz(int k, int s, int S, int M)
 {
     return k * S + (s ^ ((k / M) & (S - 1)));
@@ -152,7 +184,9 @@ __device__ __forceinline__ int lam_ix(int q, int my, int L, int ny)
     return (2 * my < ny) ? 0 : L;
 }
 
-template <int M, typename St>
+// DPP16: the P = 16 lanes of a column form one DPP row; the cyclic reduction
+// then exchanges neighbours with DPP instead of ds_bpermute + select.
+template <int M, typename St, bool DPP16>
 __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 {
     extern __shared__ double tl[];
@@ -267,6 +301,22 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
             la = 0.0; lg = 0.0; ld = 0.0;  // top value of the (0,0) mode = 0
             fg = 0.0;
         }
+        if (DPP16) {
+            // stride 1: F_p <- L_{p-1}, L_p;  L_p <- F_p, F_{p+1}
+            const double pla = dpp<ROW_SHR(1)>(la), plg = dpp<ROW_SHR(1)>(lg), pld = dpp<ROW_SHR(1)>(ld);
+            const double nfa = dpp<ROW_SHL(1)>(fa), nfg = dpp<ROW_SHL(1)>(fg), nfd = dpp<ROW_SHL(1)>(fd);
+            const double rf = rcp(1.0 - fa * plg - fg * la);
+            const double fa2 = -fa * pla * rf, fg2 = -fg * lg * rf;
+            const double fd2 = (fd - fa * pld - fg * ld) * rf;
+            const double rl = rcp(1.0 - la * fg - lg * nfa);
+            const double la2 = -la * fa * rl, lg2 = -lg * nfg * rl;
+            const double ld2 = (ld - la * fd - lg * nfd) * rl;
+            fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
+            pcr16_stage<1>(fa, fg, fd, la, lg, ld);
+            pcr16_stage<2>(fa, fg, fd, la, lg, ld);
+            pcr16_stage<4>(fa, fg, fd, la, lg, ld);
+            pcr16_stage<8>(fa, fg, fd, la, lg, ld);
+        } else {
         const int lane = tid & 63;
         // stride 1: F_p <- L_{p-1}, L_p;  L_p <- F_p, F_{p+1}
         {
@@ -307,6 +357,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
             const double la2 = -la * pla * rl, lg2 = -lg * nlg * rl;
             const double ld2 = (ld - la * pld - lg * nld) * rl;
             fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
+        }
         }
         // 3. back substitution
         const double F = fd, Lv = ld;
@@ -354,19 +405,26 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         const size_t shm = sizeof(double) * (2 * (size_t)d->nz * g.S + 3 * (size_t)d->nz);
         if (threads > 512 || shm > 163840) return -1;
         switch (d->tp.M) {
-#define PLL_TRID_CASE(M_)                                                                       \
-    case M_:                                                                                    \
-        if (shm > 65536 && hipFuncSetAttribute((const void *)k_trid_part<M_, St>,               \
+#define PLL_TRID_LAUNCH(M_, D_)                                                                 \
+    do {                                                                                        \
+        if (shm > 65536 && hipFuncSetAttribute((const void *)k_trid_part<M_, St, D_>,           \
                                                hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                                (int)shm) != hipSuccess)                         \
             return -3;                                                                          \
-        hipLaunchKernelGGL((k_trid_part<M_, St>), dim3(blocks), dim3(threads), shm, s, (St *)W, g); \
+        hipLaunchKernelGGL((k_trid_part<M_, St, D_>), dim3(blocks), dim3(threads), shm, s,      \
+                           (St *)W, g);                                                         \
+    } while (0)
+#define PLL_TRID_CASE(M_)                                                                       \
+    case M_:                                                                                    \
+        if (d->tp.dpp16) PLL_TRID_LAUNCH(M_, true);                                             \
+        else PLL_TRID_LAUNCH(M_, false);                                                        \
         break;
             PLL_TRID_CASE(4)
             PLL_TRID_CASE(8)
             PLL_TRID_CASE(12)
             PLL_TRID_CASE(16)
 #undef PLL_TRID_CASE
+#undef PLL_TRID_LAUNCH
             default: return -5;
         }
         return hipGetLastError() == hipSuccess ? 0 : -3;
