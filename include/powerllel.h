@@ -109,6 +109,21 @@ int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream);
 int pll_ppe_set_refinement(pll_ppe *h, int sweeps);
 /* Same switch for the drop-in global instance (after init_poisson_solver). */
 int set_poisson_solver_refinement(int sweeps);
+/* Refined form to a tolerance (no counterpart in the reference): sweeps
+ * until the estimated relative max-norm error of p is <= tol, at most
+ * max_sweeps (>= 1).  After sweep k the estimate is |c_1|/|p| for k = 1 and
+ * min(1, 10 |c_k|/|c_{k-1}|) |c_k|/|p| for k > 1, c_k the k-th correction.
+ * Reads two norms back per sweep (one stream synchronisation each).  The
+ * sweeps made and the final estimate go to *sweeps_done / *err_est (either
+ * may be NULL).  Work space: that of pll_ppe_solve_refined plus
+ * 2 + 2 ny nz doubles. */
+int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol,
+                              int max_sweeps, int *sweeps_done,
+                              double *err_est, void *stream);
+/* Make pll_ppe_solve use the tolerance form (pll_ppe_set_refinement
+ * switches it off), and query the sweeps and estimate of its last solve. */
+int pll_ppe_set_refinement_tol(pll_ppe *h, double tol, int max_sweeps);
+int pll_ppe_last_refinement(const pll_ppe *h, int *sweeps, double *err_est);
 /* Device bytes held by the handle. */
 size_t pll_ppe_device_bytes(const pll_ppe *h);
 int pll_ppe_destroy(pll_ppe *h);

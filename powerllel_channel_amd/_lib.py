@@ -48,6 +48,9 @@ _SIGS = {
     "pll_ppe_stage_times_mean": (_ip, [_vp, C.POINTER(_d), C.POINTER(_ip)]),
     "pll_ppe_solve_refined": (_ip, [_vp, _dp, _ip, _vp]),
     "pll_ppe_set_refinement": (_ip, [_vp, _ip]),
+    "pll_ppe_solve_refined_tol": (_ip, [_vp, _dp, _d, _ip, _vp, _vp, _vp]),
+    "pll_ppe_set_refinement_tol": (_ip, [_vp, _d, _ip]),
+    "pll_ppe_last_refinement": (_ip, [_vp, _vp, _vp]),
     "set_poisson_solver_refinement": (_ip, [_ip]),
     "pll_ppe_device_bytes": (C.c_size_t, [_vp]),
     "pll_ppe_destroy": (_ip, [_vp]),

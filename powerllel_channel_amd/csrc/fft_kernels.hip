@@ -333,6 +333,18 @@ struct SrcHalo64 {  // interior of a haloed FP64 array (the caller's p / rhs)
         return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
     }
 };
+struct SrcHalo64Copy {  // SrcHalo64, also copying the FP64 values to a dense array
+    const double *p;
+    long long off1, off2;
+    double *F;
+    template <typename V>
+    __device__ __forceinline__ V load(long long row, long long j, long long k, int m, int nx) const
+    {
+        const d2u pr = *reinterpret_cast<const d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m);
+        *reinterpret_cast<double2 *>(F + row * nx + 2 * m) = make_double2(pr.x, pr.y);
+        return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
+    }
+};
 struct SrcDense32 {  // dense FP32 rows (the residual of a refinement sweep)
     const float *r;
     template <typename V>
@@ -446,6 +458,43 @@ struct DstHaloFromDense {  // haloed FP64 interior := dense FP64 + value - eps
     }
 };
 
+// Destinations that also report max |result| of the pair (store_n); the
+// kernel adds max |value| and reduces both per workgroup.
+__device__ __forceinline__ double nmax(double m, double x)
+{
+    return (x > m || x != x) ? x : m;  // NaN wins
+}
+struct DstHaloSetN {  // haloed FP64 interior := value
+    double *p;
+    long long off1, off2;
+    __device__ __forceinline__ double store_n(long long, long long j, long long k, int m, int,
+                                              double a, double b) const
+    {
+        d2u pr;
+        pr.x = a;
+        pr.y = b;
+        *reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m) = pr;
+        return nmax(fabs(a), fabs(b));
+    }
+};
+struct DstHaloAddN {  // haloed FP64 interior := (interior - eps) + value
+    double *p;
+    long long off1, off2;
+    const double *eps;  // gauge constant (device scalar), may be null
+    __device__ __forceinline__ double store_n(long long, long long j, long long k, int m, int,
+                                              double a, double b) const
+    {
+        d2u *q = reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m);
+        const d2u o = *q;
+        const double e = eps ? *eps : 0.0;
+        d2u pr;
+        pr.x = (o.x - e) + a;
+        pr.y = (o.y - e) + b;
+        *q = pr;
+        return nmax(fabs(pr.x), fabs(pr.y));
+    }
+};
+
 template <typename V>
 struct XArgs {
     pll_fftplan P;
@@ -453,6 +502,7 @@ struct XArgs {
     long long nrows;    // ny*nz
     int nx, ny, rows_per_wg, pitch;
     double norm;
+    double *norm_part;  // NORM kernels: [2 * workgroups] (max |value|, max |result|)
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin; renumbering them so
@@ -528,8 +578,9 @@ k_xfwd(Src src, V *__restrict__ W, XArgs<V> a)
     }
 }
 
-// x backward: packed half spectrum rows -> real rows * norm, stored by Dst.
-template <class F, typename V, class Dst>
+// x backward: packed half spectrum rows -> real rows * norm, stored by Dst;
+// with NORM, also the workgroup's max |value| and max |result|.
+template <class F, typename V, class Dst, bool NORM = false>
 __global__ void __launch_bounds__(max_threads_f<F>())
 k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
 {
@@ -568,12 +619,73 @@ k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
     }
     __syncthreads();
     F::template run<+1, 2>(v, t, a.P, a.tw, ml);
-    if (valid) {
+    if constexpr (!NORM) {
+        if (valid) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int m = t + e * T;
-            dst.store(row, j, k, m, a.nx, (double)v[e].x * a.norm, (double)v[e].y * a.norm);
+            for (int e = 0; e < E; ++e) {
+                const int m = t + e * T;
+                dst.store(row, j, k, m, a.nx, (double)v[e].x * a.norm, (double)v[e].y * a.norm);
+            }
         }
+    } else {
+        double nc = 0.0, np = 0.0;
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int m = t + e * T;
+                const double x0 = (double)v[e].x * a.norm, x1 = (double)v[e].y * a.norm;
+                np = nmax(np, dst.store_n(row, j, k, m, a.nx, x0, x1));
+                nc = nmax(nc, nmax(fabs(x0), fabs(x1)));
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            nc = nmax(nc, __shfl_xor(nc, o));
+            np = nmax(np, __shfl_xor(np, o));
+        }
+        __syncthreads();  // the FFT's last LDS reads are done
+        double *red = reinterpret_cast<double *>(lds_raw);
+        const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[2 * wv] = nc;
+            red[2 * wv + 1] = np;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < nw; ++q) {
+                nc = nmax(nc, red[2 * q]);
+                np = nmax(np, red[2 * q + 1]);
+            }
+            a.norm_part[2 * blockIdx.x] = nc;
+            a.norm_part[2 * blockIdx.x + 1] = np;
+        }
+    }
+}
+
+// out[0..1] = max over the n partial pairs (fixed order)
+__global__ void __launch_bounds__(256) k_norm_final(const double *__restrict__ part, long long n,
+                                                    double *__restrict__ out)
+{
+    __shared__ double red[512];
+    const int t = threadIdx.x;
+    double nc = 0.0, np = 0.0;
+    for (long long q = t; q < n; q += 256) {
+        nc = nmax(nc, part[2 * q]);
+        np = nmax(np, part[2 * q + 1]);
+    }
+    red[2 * t] = nc;
+    red[2 * t + 1] = np;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            red[2 * t] = nmax(red[2 * t], red[2 * (t + h)]);
+            red[2 * t + 1] = nmax(red[2 * t + 1], red[2 * (t + h) + 1]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = red[0];
+        out[1] = red[1];
     }
 }
 
@@ -709,17 +821,30 @@ int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-template <class F, typename V, class Dst>
-int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipStream_t s)
+template <class F, typename V, class Dst, bool NORM = false>
+int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipStream_t s,
+              double *norm = nullptr)
 {
-    const XArgs<V> a = x_args<V>(d, tw);
+    XArgs<V> a = x_args<V>(d, tw);
     const int threads = a.rows_per_wg * a.P.T;
     const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(V);
     if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
-    hipLaunchKernelGGL((k_xbwd<F, V, Dst>), dim3((unsigned)blocks), dim3(threads), shm, s,
+    if (NORM) {
+        // norm[0..1]: result; norm[2 ..): partials, room for ny*nz workgroups
+        if (!norm || blocks > a.nrows || shm < sizeof(double) * 2 * ((threads + 63) / 64))
+            return -1;
+        a.norm_part = norm + 2;
+    }
+    hipLaunchKernelGGL((k_xbwd<F, V, Dst, NORM>), dim3((unsigned)blocks), dim3(threads), shm, s,
                        (const V *)W, dst, a);
-    return hipGetLastError() == hipSuccess ? 0 : -3;
+    if (hipGetLastError() != hipSuccess) return -3;
+    if (NORM) {
+        hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, s, (const double *)(norm + 2),
+                           blocks, norm);
+        if (hipGetLastError() != hipSuccess) return -3;
+    }
+    return 0;
 }
 
 template <class F, typename V>
@@ -896,6 +1021,31 @@ extern "C" int pll_launch_xbwd32(const pll_ppe_dev *d, int dst_kind, const void 
                                             DstHaloFromDense{p_halo, off1, off2, P_dense, eps}, s);
             default: return -1;
         }
+    });
+}
+
+extern "C" int pll_launch_xfwd32_copy(const pll_ppe_dev *d, const double *f_halo, double *F,
+                                      void *W32, hipStream_t s)
+{
+    const SrcHalo64Copy in{f_halo, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2, F};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xf<decltype(f), float2>(d, d->tw_x32, in, W32, s);
+    });
+}
+
+extern "C" int pll_launch_xbwd32_norm(const pll_ppe_dev *d, int add, const void *W32,
+                                      double *p_halo, const double *eps, double *norm,
+                                      hipStream_t s)
+{
+    const long long off1 = (long long)(d->nx + 2) * (d->ny + 2), off2 = d->nx + 2;
+    return dispatch_plan(d->px, [&](auto f) {
+        using F = decltype(f);
+        if (add)
+            return launch_xb<F, float2, DstHaloAddN, true>(d, d->tw_x32, W32,
+                                                            DstHaloAddN{p_halo, off1, off2, eps},
+                                                            s, norm);
+        return launch_xb<F, float2, DstHaloSetN, true>(d, d->tw_x32, W32,
+                                                        DstHaloSetN{p_halo, off1, off2}, s, norm);
     });
 }
 

@@ -7,8 +7,8 @@
 // src/src_c/mod_fft_fftw.inc:180-183 are exactly those of the periodic second
 // difference) and the wall-closed z coefficients of set_trid_coeff
 // (src/src_c/mod_poisson_solver.c:40-103), where a_0 = c_{nz-1} = 0.
-// f is the interior of the caller's haloed array, p a dense array (x fastest);
-// r is written in FP32 for the FP32 correction solve.
+// f and p are each the interior of a haloed array or a dense array (x
+// fastest); r is written in FP32 for the FP32 correction solve.
 //
 // Every term is formed from neighbour differences, with the z diagonal split
 // as b_k = d_k - a_k - c_k (d_k the row sum, zero up to the rounding of b):
@@ -39,15 +39,21 @@ namespace {
 
 constexpr int TX = 64, TY = 4, KC = 16;
 
+// Element (i, j, k) of a field at base + k s1 + j s2 + i: a haloed array
+// (base = s1 + s2 + 1, s1 = (nx+2)(ny+2), s2 = nx+2) or a dense one (0, nx ny, nx).
+struct Lay {
+    long long base, s1, s2;
+};
+
 struct RArgs {
-    const double *f;   // haloed
-    const double *P;   // dense
+    const double *f;   // right-hand side
+    const double *P;   // current solution
+    Lay fl, pl;
     float *r;          // dense
     double *gpart;     // non-null: per-block sums of P over k = nz-1
     const double *a, *c, *d;
     double idx2, idy2;
     int nx, ny, nz;
-    long long off1, off2;
 };
 
 __global__ void __launch_bounds__(TX *TY) k_resid(RArgs g)
@@ -63,17 +69,20 @@ __global__ void __launch_bounds__(TX *TY) k_resid(RArgs g)
         const int ip = i + 1 == g.nx ? 0 : i + 1, im = i == 0 ? g.nx - 1 : i - 1;
         const int jp = j + 1 == g.ny ? 0 : j + 1, jm = j == 0 ? g.ny - 1 : j - 1;
         const long long cidx = (long long)j * g.nx + i;
-        const double *__restrict__ P = g.P;
-        double pm = k0 > 0 ? P[(k0 - 1) * plane + cidx] : 0.0;
-        double pc = P[k0 * plane + cidx];
+        const long long s1 = g.pl.s1, s2 = g.pl.s2;
+        const double *__restrict__ P = g.P + g.pl.base;  // P[k s1 + j s2 + i]
+        const double *__restrict__ F = g.f + g.fl.base;
+        const long long pc0 = (long long)j * s2 + i;
+        double pm = k0 > 0 ? P[(k0 - 1) * s1 + pc0] : 0.0;
+        double pc = P[k0 * s1 + pc0];
         for (int k = k0; k < k1; ++k) {
-            const long long kb = (long long)k * plane;
-            const double pp = k + 1 < g.nz ? P[kb + plane + cidx] : 0.0;
-            const double lx = ((P[kb + (long long)j * g.nx + ip] - pc) + (P[kb + (long long)j * g.nx + im] - pc)) * g.idx2;
-            const double ly = ((P[kb + (long long)jp * g.nx + i] - pc) + (P[kb + (long long)jm * g.nx + i] - pc)) * g.idy2;
+            const long long kb = (long long)k * s1;
+            const double pp = k + 1 < g.nz ? P[kb + s1 + pc0] : 0.0;
+            const double lx = ((P[kb + (long long)j * s2 + ip] - pc) + (P[kb + (long long)j * s2 + im] - pc)) * g.idx2;
+            const double ly = ((P[kb + (long long)jp * s2 + i] - pc) + (P[kb + (long long)jm * s2 + i] - pc)) * g.idy2;
             const double lz = g.a[k] * (pm - pc) + g.c[k] * (pp - pc) + g.d[k] * pc;
-            const double f = g.f[(k + 1) * g.off1 + (j + 1) * g.off2 + i + 1];
-            g.r[kb + cidx] = (float)(f - (lx + ly + lz));
+            const double f = F[k * g.fl.s1 + (long long)j * g.fl.s2 + i];
+            g.r[k * plane + cidx] = (float)(f - (lx + ly + lz));
             if (k == g.nz - 1) top = pc;
             pm = pc;
             pc = pp;
@@ -116,12 +125,25 @@ extern "C" int pll_gauge_parts(const pll_ppe_dev *d)
     return ((d->nx + TX - 1) / TX) * ((d->ny + TY - 1) / TY);
 }
 
-extern "C" int pll_launch_resid(const pll_ppe_dev *d, const double *f_halo, const double *P,
-                                float *r, double *gauge, hipStream_t s)
+static Lay halo_lay(const pll_ppe_dev *d)
+{
+    const long long s1 = (long long)(d->nx + 2) * (d->ny + 2), s2 = d->nx + 2;
+    return Lay{s1 + s2 + 1, s1, s2};
+}
+
+static Lay dense_lay(const pll_ppe_dev *d)
+{
+    return Lay{0, (long long)d->nx * d->ny, d->nx};
+}
+
+static int launch_resid(const pll_ppe_dev *d, const double *f, Lay fl, const double *P, Lay pl,
+                        float *r, double *gauge, hipStream_t s)
 {
     RArgs g;
-    g.f = f_halo;
+    g.f = f;
     g.P = P;
+    g.fl = fl;
+    g.pl = pl;
     g.r = r;
     g.gpart = gauge;  // gauge[0 .. parts): partials, gauge[parts]: the mean
     g.a = d->a;
@@ -132,8 +154,6 @@ extern "C" int pll_launch_resid(const pll_ppe_dev *d, const double *f_halo, cons
     g.nx = d->nx;
     g.ny = d->ny;
     g.nz = d->nz;
-    g.off1 = (long long)(d->nx + 2) * (d->ny + 2);
-    g.off2 = d->nx + 2;
     const dim3 grid((d->nx + TX - 1) / TX, (d->ny + TY - 1) / TY, (d->nz + KC - 1) / KC);
     hipLaunchKernelGGL(k_resid, grid, dim3(TX, TY), 0, s, g);
     if (hipGetLastError() != hipSuccess) return -3;
@@ -144,4 +164,16 @@ extern "C" int pll_launch_resid(const pll_ppe_dev *d, const double *f_halo, cons
         if (hipGetLastError() != hipSuccess) return -3;
     }
     return 0;
+}
+
+extern "C" int pll_launch_resid(const pll_ppe_dev *d, const double *f_halo, const double *P,
+                                float *r, double *gauge, hipStream_t s)
+{
+    return launch_resid(d, f_halo, halo_lay(d), P, dense_lay(d), r, gauge, s);
+}
+
+extern "C" int pll_launch_resid_hp(const pll_ppe_dev *d, const double *F, const double *p_halo,
+                                   float *r, double *gauge, hipStream_t s)
+{
+    return launch_resid(d, F, dense_lay(d), p_halo, halo_lay(d), r, gauge, s);
 }

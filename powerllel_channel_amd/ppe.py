@@ -111,6 +111,32 @@ class PoissonSolver:
         """Make solve() use the refined form with `sweeps` sweeps (-1: FP64)."""
         check(lib().pll_ppe_set_refinement(self._h, int(sweeps)))
 
+    def solve_refined_tol(self, p, tol: float, max_sweeps: int = 8, stream=None):
+        """Refined form, sweeping until the estimated relative max-norm error
+        of p is <= tol (at most max_sweeps).  Returns (sweeps, estimate);
+        synchronises `stream` once per sweep."""
+        if self._h is None:
+            raise RuntimeError("solver was closed")
+        if tuple(p.shape) != self.shape:
+            raise ValueError(f"p has shape {tuple(p.shape)}, expected {self.shape}")
+        addr, dev = _ptr_and_kind(p)
+        if not dev:
+            raise ValueError("PoissonSolver.solve_refined_tol takes a device tensor")
+        n, est = C.c_int(), C.c_double()
+        check(lib().pll_ppe_solve_refined_tol(self._h, addr, float(tol), int(max_sweeps),
+                                              C.byref(n), C.byref(est), _stream_ptr(stream)))
+        return n.value, est.value
+
+    def set_refinement_tol(self, tol: float, max_sweeps: int = 8):
+        """Make solve() refine to `tol` (set_refinement switches it off)."""
+        check(lib().pll_ppe_set_refinement_tol(self._h, float(tol), int(max_sweeps)))
+
+    def last_refinement(self):
+        """(sweeps, error estimate) of the last tolerance-form solve."""
+        n, est = C.c_int(), C.c_double()
+        check(lib().pll_ppe_last_refinement(self._h, C.byref(n), C.byref(est)))
+        return n.value, est.value
+
     def set_timing(self, on: bool = True):
         check(lib().pll_ppe_set_timing(self._h, 1 if on else 0))
 

@@ -198,7 +198,7 @@ def _directions(torch, cuda, g, n, seed):
     return out
 
 
-def _fd_check(torch, cuda, g, ch, s, nsteps, every, ndir, eps, tol):
+def _fd_check(torch, cuda, g, ch, s, nsteps, every, ndir, eps, tol, report_eps=None):
     from powerllel_channel_amd.adjoint import WallShearWindow
     U0, B = _window_setup(torch, cuda, g)
     ch.bc_vel(*U0)
@@ -215,6 +215,11 @@ def _fd_check(torch, cuda, g, ch, s, nsteps, every, ndir, eps, tol):
         rel = abs(fd - ad) / abs(fd)
         worst = max(worst, rel)
         print(f"  J {J:.12e}  FD {fd:.12e}  adjoint {ad:.12e}  rel {rel:.1e}")
+        if report_eps:
+            Jp = win.forward(*U0, *[b + report_eps * x for b, x in zip(B, d)])
+            Jm = win.forward(*U0, *[b - report_eps * x for b, x in zip(B, d)])
+            fd2 = (Jp - Jm) / (2 * report_eps)
+            print(f"    eps {report_eps:.0e}: FD {fd2:.12e}  rel {abs(fd2 - ad) / abs(fd2):.1e}")
     win.close()
     assert worst <= tol, worst
     return grad
@@ -317,4 +322,5 @@ def test_window_gradient_c4(cuda):
     ch = _channel(g)
     s = PoissonSolver(g.nx, g.ny, g.nz, g.dx, g.dy, g.dzf)
     print("[adjoint window] 256x192x128, 20 steps, checkpoints every 5", flush=True)
-    _fd_check(torch, cuda, g, ch, s, nsteps=20, every=5, ndir=16, eps=1e-2, tol=1e-6)
+    _fd_check(torch, cuda, g, ch, s, nsteps=20, every=5, ndir=16, eps=1e-2, tol=1e-6,
+              report_eps=1e-6)

@@ -22,6 +22,9 @@ CASES = [
     (128, 96, 32, 4.4),
     (192, 192, 128, 3.0),
     (256, 192, 128, 3.0),
+    # transform lengths of the 1536x1024x384 grid: x 768 (nx/2), y 1024; y 768
+    (1536, 1024, 8, 4.4),
+    (64, 768, 16, 4.4),
 ]
 
 
@@ -177,7 +180,7 @@ REFINED_BAR = {0: 1e-5, 1: 1e-9, 2: 1e-11, 3: 5e-12}
 
 
 @pytest.mark.parametrize("nx,ny,nz,sr", [(768, 384, 256, 4.0), (192, 192, 128, 3.0),
-                                         (48, 36, 24, 3.0)])
+                                         (48, 36, 24, 3.0), (1536, 1024, 8, 4.4)])
 def test_refined_error_per_sweep(cuda, nx, ny, nz, sr):
     import torch
     from powerllel_channel_amd import PoissonSolver
@@ -212,6 +215,44 @@ def test_refined_error_per_sweep(cuda, nx, ny, nz, sr):
           ", ".join(f"k={k}: {errs[k]:.2e} / {errs_mr[k]:.2e}" for k in errs))
     for k, e in errs.items():
         assert e <= REFINED_BAR[k], (k, e)
+
+
+@pytest.mark.parametrize("nx,ny,nz,sr", [(768, 384, 256, 4.0), (192, 192, 128, 3.0),
+                                         (1536, 1024, 8, 4.4)])
+def test_refined_to_tolerance(cuda, nx, ny, nz, sr):
+    """The tolerance form stops at an error (against the FP64 solve) at or
+    below the tolerance, with the estimate it reports also below it; the
+    tolerance switch drives solve()."""
+    import torch
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, spectral_rhs
+    print(f"[refined tol] {nx}x{ny}x{nz} sr={sr}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, sr)
+    rhs = spectral_rhs(g, 13)
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    ref = torch.from_numpy(rhs).to(cuda)
+    s.solve(ref)
+    inner = (slice(1, -1),) * 3
+    scale = float(ref[inner].abs().max())
+    for tol in (1e-6, 1e-9, 1e-11):
+        p = torch.from_numpy(rhs).to(cuda)
+        n, est = s.solve_refined_tol(p, tol, max_sweeps=8)
+        err = float((p[inner] - ref[inner]).abs().max()) / scale
+        mask = torch.ones_like(p, dtype=torch.bool)
+        mask[inner] = False
+        assert torch.equal(p[mask], ref[mask]), "halos must be untouched"
+        print(f"  tol {tol:.0e}: {n} sweeps, estimate {est:.2e}, error {err:.2e}")
+        assert est <= tol and err <= tol, (tol, n, est, err)
+    s.set_refinement_tol(1e-11, 8)
+    p2 = torch.from_numpy(rhs).to(cuda)
+    s.solve(p2)
+    assert s.last_refinement() == (n, est)
+    assert torch.equal(p2, p)
+    s.set_refinement(-1)
+    q = torch.from_numpy(rhs).to(cuda)
+    s.solve(q)
+    assert torch.equal(q, ref)
 
 
 def test_refined_vs_oracle_and_dropin(orc, cuda):
