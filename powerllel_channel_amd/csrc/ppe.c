@@ -226,7 +226,8 @@ static int part_fits(int nx, int nz, int m, int *M, int *P, int *S)
     if (p < 2 || p > 64 || (p & (p - 1))) return 0;
     for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
         const int sc = Ss[j];
-        if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 + (size_t)nz * 24 > 163840) continue;
+        if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 + (size_t)nz * 24 + (size_t)p * 8 > 163840)
+            continue;
         *M = m;
         *P = p;
         *S = sc;

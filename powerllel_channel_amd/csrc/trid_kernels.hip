@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_trid(C *__restrict__ W, TArgs g)
 struct PArgs {
     const double *a, *c, *bz, *lamx, *lamy, *piv;
     const int *order;
-    int L, nx, ny, nz, P, S, ntx, pin_gauge;
+    int L, nx, ny, nz, P, lP, S, ntx, pin_gauge;  // P = 1 << lP
     int qx, iyx;  // exact path for tiles with q0 < qx and iy < iyx
 };
 
@@ -186,33 +186,40 @@ __device__ __forceinline__ int lam_ix(int q, int my, int L, int ny)
 
 // DPP16: the P = 16 lanes of a column form one DPP row; the cyclic reduction
 // then exchanges neighbours with DPP instead of ds_bpermute + select.
-template <int M, typename St, bool DPP16>
+template <int M, int S, typename St, bool DPP16>
 __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 {
     extern __shared__ double tl[];
-    const int S = g.S, P = g.P, nz = g.nz;
+    const int lP = g.lP, P = 1 << lP, nz = g.nz;
+    constexpr int LS = S == 16 ? 4 : (S == 8 ? 3 : 2);
+    static_assert(S == 4 || S == 8 || S == 16, "columns per tile");
     const int tid = threadIdx.x;
     const int tile = g.order[blockIdx.x];
     const int my = tile / g.ntx, q0 = (tile % g.ntx) * S;
     const long long plane = (long long)g.nx * g.ny;
     St *__restrict__ base = Wr + (long long)my * g.nx + q0;
     const int nthr = S * P;
-    // z coefficients staged in LDS after the tile and its pivot half, as
-    // [a | c | bz] each in [i][p] order (row k = p*M + i at i*P + p), so the
-    // P lanes of a column read consecutive words
+    // z coefficients staged in LDS after the tile and its pivot half: row
+    // k = p M + i of (a, c, bz) at cf[p CS + 3 i + {0, 1, 2}].  The odd pitch
+    // CS = 3M + 1 puts the P lanes of a column in distinct banks, and a lane's
+    // reads are at fixed offsets from one address.
+    constexpr int CS = 3 * M + 1;
     double *cf = tl + 2 * (size_t)nz * S;
 
+    // element j of this thread: column st, row kt + j P (nthr = S P)
+    const int st = tid & (S - 1), kt = tid >> LS;
+    const St *__restrict__ src = base + st + kt * plane;
+    const long long jstride = (long long)P * plane;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-        const int e = tid + j * nthr;
-        const int k = e / S, s = e - k * S;
-        tl[swz(k, s, S, M)] = (double)base[k * plane + s];
+        const int k = kt + (j << lP);
+        tl[swz(k, st, S, M)] = (double)src[j * jstride];
     }
     for (int k = tid; k < nz; k += nthr) {
-        const int o = (k % M) * P + k / M;
+        const int o = (k / M) * CS + 3 * (k % M);
         cf[o] = g.a[k];
-        cf[nz + o] = g.c[k];
-        cf[2 * nz + o] = g.bz[k];
+        cf[o + 1] = g.c[k];
+        cf[o + 2] = g.bz[k];
     }
     __syncthreads();
 
@@ -235,7 +242,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 #pragma unroll 4
             for (int k = 1; k < nz; ++k) {
                 const int o = swz(k, s, S, M);
-                const double m = cf[(k % M) * P + k / M] / pt[swz(k - 1, s, S, M)];
+                const double m = cf[(k / M) * CS + 3 * (k % M)] / pt[swz(k - 1, s, S, M)];
                 const double cur = tl[o] - m * prev;
                 tl[o] = cur;
                 prev = cur;
@@ -247,23 +254,26 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
 #pragma unroll 4
             for (int k = nz - 2; k >= 0; --k) {
                 const int o = swz(k, s, S, M);
-                const double cur = (tl[o] - cf[nz + (k % M) * P + k / M] * prev) / pt[o];
+                const double cur = (tl[o] - cf[(k / M) * CS + 3 * (k % M) + 1] * prev) / pt[o];
                 tl[o] = cur;
                 prev = cur;
             }
         }
     } else {
-        const int s = tid / P, p = tid - s * P;
+        const int s = tid >> lP, p = tid & (P - 1);
         const int q = q0 + s;
         const double lam = g.lamx[lam_ix(q, my, g.L, g.ny)] + g.lamy[iy];
         const int k0 = p * M;
+        // swz(k0 + i, s) = pb + i S for i < M, since (k0 + i) / M = p
+        const int pb = k0 * S + (s ^ (p & (S - 1)));
         double d[M], ap[M], cp[M];
 #pragma unroll
-        for (int i = 0; i < M; ++i) d[i] = tl[swz(k0 + i, s, S, M)];
+        for (int i = 0; i < M; ++i) d[i] = tl[pb + i * S];
         // 1. local elimination
-#define CA(i) cf[(i) * P + p]
-#define CC(i) cf[nz + (i) * P + p]
-#define CB(i) cf[2 * nz + (i) * P + p]
+        const double *__restrict__ cl = cf + p * CS;
+#define CA(i) cl[3 * (i)]
+#define CC(i) cl[3 * (i) + 1]
+#define CB(i) cl[3 * (i) + 2]
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const double r = rcp(CB(i) + lam);
@@ -366,14 +376,14 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
         d[0] = F;
         d[M - 1] = Lv;
 #pragma unroll
-        for (int i = 0; i < M; ++i) tl[swz(k0 + i, s, S, M)] = d[i];
+        for (int i = 0; i < M; ++i) tl[pb + i * S] = d[i];
     }
     __syncthreads();
+    St *__restrict__ dst = base + st + kt * plane;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-        const int e = tid + j * nthr;
-        const int k = e / S, s = e - k * S;
-        base[k * plane + s] = (St)tl[swz(k, s, S, M)];
+        const int k = kt + (j << lP);
+        dst[j * jstride] = (St)tl[swz(k, st, S, M)];
     }
 }
 
@@ -394,6 +404,9 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         g.ny = d->ny;
         g.nz = d->nz;
         g.P = d->tp.P;
+        g.lP = 0;
+        while ((1 << g.lP) < g.P) g.lP++;
+        if ((1 << g.lP) != g.P) return -1;
         g.S = d->tp.S;
         g.ntx = d->nx / d->tp.S;
         g.pin_gauge = d->pin_gauge;
@@ -401,29 +414,40 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         g.iyx = d->tp.iyx;
         const unsigned blocks = (unsigned)(g.ntx * d->ny);
         const int threads = g.S * g.P;
-        // values + pivots + coefficients
-        const size_t shm = sizeof(double) * (2 * (size_t)d->nz * g.S + 3 * (size_t)d->nz);
+        // values + pivots + coefficients (P rows of 3M + 1)
+        const size_t shm = sizeof(double) * (2 * (size_t)d->nz * g.S + 3 * (size_t)d->nz + g.P);
         if (threads > 512 || shm > 163840) return -1;
         switch (d->tp.M) {
-#define PLL_TRID_LAUNCH(M_, D_)                                                                 \
+#define PLL_TRID_LAUNCH(M_, S_, D_)                                                             \
     do {                                                                                        \
-        if (shm > 65536 && hipFuncSetAttribute((const void *)k_trid_part<M_, St, D_>,           \
+        if (shm > 65536 && hipFuncSetAttribute((const void *)k_trid_part<M_, S_, St, D_>,       \
                                                hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                                (int)shm) != hipSuccess)                         \
             return -3;                                                                          \
-        hipLaunchKernelGGL((k_trid_part<M_, St, D_>), dim3(blocks), dim3(threads), shm, s,      \
+        hipLaunchKernelGGL((k_trid_part<M_, S_, St, D_>), dim3(blocks), dim3(threads), shm, s,  \
                            (St *)W, g);                                                         \
     } while (0)
+#define PLL_TRID_S(M_, D_)                                                                      \
+    switch (g.S) {                                                                              \
+        case 4: PLL_TRID_LAUNCH(M_, 4, D_); break;                                              \
+        case 8: PLL_TRID_LAUNCH(M_, 8, D_); break;                                              \
+        case 16: PLL_TRID_LAUNCH(M_, 16, D_); break;                                            \
+        default: return -5;                                                                     \
+    }
 #define PLL_TRID_CASE(M_)                                                                       \
     case M_:                                                                                    \
-        if (d->tp.dpp16) PLL_TRID_LAUNCH(M_, true);                                             \
-        else PLL_TRID_LAUNCH(M_, false);                                                        \
+        if (d->tp.dpp16) {                                                                      \
+            PLL_TRID_S(M_, true)                                                                \
+        } else {                                                                                \
+            PLL_TRID_S(M_, false)                                                               \
+        }                                                                                       \
         break;
             PLL_TRID_CASE(4)
             PLL_TRID_CASE(8)
             PLL_TRID_CASE(12)
             PLL_TRID_CASE(16)
 #undef PLL_TRID_CASE
+#undef PLL_TRID_S
 #undef PLL_TRID_LAUNCH
             default: return -5;
         }
