@@ -20,6 +20,7 @@
 // its last pivot is exactly zero; solutions differ from the reference's only
 // by an additive constant.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include "pll_internal.h"
 
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(256) k_trid(C *__restrict__ W, TArgs g)
 struct PArgs {
     const double *a, *c, *bz, *lamx, *lamy, *piv;
     const int *order;
-    int L, nx, ny, nz, P, lP, S, ntx, pin_gauge;  // P = 1 << lP
+    int L, nx, ny, nz, P, lP, S, ntx, ntiles, pin_gauge;  // P = 1 << lP
     int qx, iyx;  // exact path for tiles with q0 < qx and iy < iyx
 };
 
@@ -184,20 +185,23 @@ __device__ __forceinline__ int lam_ix(int q, int my, int L, int ny)
     return (2 * my < ny) ? 0 : L;
 }
 
+// Persistent: workgroup b solves tiles order[b], order[b + G], ... (G the
+// grid), loading the next tile's values into registers while it solves the
+// current one, so the HBM reads of one tile overlap the arithmetic of the
+// previous.  With G a multiple of 8, position t of the order still runs on
+// XCD t % 8.
 // DPP16: the P = 16 lanes of a column form one DPP row; the cyclic reduction
 // then exchanges neighbours with DPP instead of ds_bpermute + select.
 template <int M, int S, typename St, bool DPP16>
-__global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
+__global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
+                                                   const int *__restrict__ order, PArgs g)
 {
     extern __shared__ double tl[];
     const int lP = g.lP, P = 1 << lP, nz = g.nz;
     constexpr int LS = S == 16 ? 4 : (S == 8 ? 3 : 2);
     static_assert(S == 4 || S == 8 || S == 16, "columns per tile");
     const int tid = threadIdx.x;
-    const int tile = g.order[blockIdx.x];
-    const int my = tile / g.ntx, q0 = (tile % g.ntx) * S;
     const long long plane = (long long)g.nx * g.ny;
-    St *__restrict__ base = Wr + (long long)my * g.nx + q0;
     const int nthr = S * P;
     // z coefficients staged in LDS after the tile and its pivot half: row
     // k = p M + i of (a, c, bz) at cf[p CS + 3 i + {0, 1, 2}].  The odd pitch
@@ -206,24 +210,56 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
     constexpr int CS = 3 * M + 1;
     double *cf = tl + 2 * (size_t)nz * S;
 
-    // element j of this thread: column st, row kt + j P (nthr = S P)
-    const int st = tid & (S - 1), kt = tid >> LS;
-    const St *__restrict__ src = base + st + kt * plane;
-    const long long jstride = (long long)P * plane;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        const int k = kt + (j << lP);
-        tl[swz(k, st, S, M)] = (double)src[j * jstride];
-    }
     for (int k = tid; k < nz; k += nthr) {
         const int o = (k / M) * CS + 3 * (k % M);
         cf[o] = g.a[k];
         cf[o + 1] = g.c[k];
         cf[o + 2] = g.bz[k];
     }
-    __syncthreads();
-
+    // element j of this thread: column st, row kt + j P (nthr = S P), at
+    // tile_base + toff + j jstride
+    const int st = tid & (S - 1), kt = tid >> LS;
+    const long long toff = st + kt * plane, jstride = (long long)P * plane;
+    St pre[M];
+    auto fetch = [&](int tile) {
+        const St *__restrict__ src =
+            Wr + (long long)(tile / g.ntx) * g.nx + (tile % g.ntx) * S + toff;
+#pragma unroll
+        for (int j = 0; j < M; ++j) pre[j] = src[j * jstride];
+    };
+    int tile = (int)blockIdx.x < g.ntiles ? order[blockIdx.x] : 0;
+    if ((int)blockIdx.x < g.ntiles) fetch(tile);
+    // Results of the previous tile, held in registers until the current
+    // tile's values are in LDS: stores and loads share one in-order counter,
+    // so a store issued before that wait would be waited for with it.
+    St out[M];
+    St *__restrict__ out_base = nullptr;
+    // Every global load of an iteration is issued before the prefetch of the
+    // next tile, and the prefetch is unconditional (the last iteration
+    // re-reads its own tile), so that the wait for a load never includes the
+    // prefetch.
+    for (int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const int tn = t + (int)gridDim.x;
+    const int tile_next = order[tn < g.ntiles ? tn : t];
+    const int my = tile / g.ntx, q0 = (tile % g.ntx) * S;
+    St *__restrict__ base = Wr + (long long)my * g.nx + q0;
     const int iy = my <= g.ny - my ? my : g.ny - my;
+    const int s = tid >> lP, p = tid & (P - 1);
+    const int q = q0 + s;
+    const double lam = g.lamx[lam_ix(q, my, g.L, g.ny)] + g.lamy[iy];
+    __syncthreads();  // the previous tile's results have been read from LDS
+#pragma unroll
+    for (int j = 0; j < M; ++j) tl[swz(kt + (j << lP), st, S, M)] = (double)pre[j];
+    __syncthreads();
+    // lam is needed from here on: waiting for it now, before the stores and
+    // the prefetch are issued, keeps those out of that wait
+    asm volatile("" : : "v"(lam));
+    if (out_base) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) out_base[j * jstride] = out[j];
+    }
+    fetch(tile_next);
+
     if (q0 < g.qx && iy < g.iyx) {
         // stored pivots of the tile's columns into the second half of LDS
         double *pt = tl + nz * S;
@@ -260,9 +296,6 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
             }
         }
     } else {
-        const int s = tid >> lP, p = tid & (P - 1);
-        const int q = q0 + s;
-        const double lam = g.lamx[lam_ix(q, my, g.L, g.ny)] + g.lamy[iy];
         const int k0 = p * M;
         // swz(k0 + i, s) = pb + i S for i < M, since (k0 + i) / M = p
         const int pb = k0 * S + (s ^ (p & (S - 1)));
@@ -379,12 +412,40 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr, PArgs g)
         for (int i = 0; i < M; ++i) tl[pb + i * S] = d[i];
     }
     __syncthreads();
-    St *__restrict__ dst = base + st + kt * plane;
 #pragma unroll
-    for (int j = 0; j < M; ++j) {
-        const int k = kt + (j << lP);
-        dst[j * jstride] = (St)tl[swz(k, st, S, M)];
+    for (int j = 0; j < M; ++j) out[j] = (St)tl[swz(kt + (j << lP), st, S, M)];
+    out_base = base + toff;
+    tile = tile_next;
     }
+    if (out_base) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) out_base[j * jstride] = out[j];
+    }
+}
+
+// Workgroups of a persistent launch: as many as are co-resident (PLL_TRID_WG
+// per CU overrides), a multiple of 8, at most one per tile.
+unsigned persistent_grid(const void *fn, int threads, size_t shm, int ntiles)
+{
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu <= 0)
+            ncu = 256;
+    }
+    int per_cu = 0;
+    const char *e = getenv("PLL_TRID_WG");
+    if (e) per_cu = atoi(e);
+    if (per_cu <= 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, shm) != hipSuccess ||
+         per_cu <= 0))
+        per_cu = 1;
+    long long n = (long long)per_cu * ncu;
+    n -= n % 8;
+    if (n < 8) n = 8;
+    return (unsigned)(n < ntiles ? n : ntiles);
 }
 
 template <typename St, typename C>
@@ -409,10 +470,10 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         if ((1 << g.lP) != g.P) return -1;
         g.S = d->tp.S;
         g.ntx = d->nx / d->tp.S;
+        g.ntiles = g.ntx * d->ny;
         g.pin_gauge = d->pin_gauge;
         g.qx = d->tp.qx;
         g.iyx = d->tp.iyx;
-        const unsigned blocks = (unsigned)(g.ntx * d->ny);
         const int threads = g.S * g.P;
         // values + pivots + coefficients (P rows of 3M + 1)
         const size_t shm = sizeof(double) * (2 * (size_t)d->nz * g.S + 3 * (size_t)d->nz + g.P);
@@ -424,8 +485,10 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
                                                hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                                (int)shm) != hipSuccess)                         \
             return -3;                                                                          \
+        const unsigned blocks =                                                                 \
+            persistent_grid((const void *)k_trid_part<M_, S_, St, D_>, threads, shm, g.ntiles); \
         hipLaunchKernelGGL((k_trid_part<M_, S_, St, D_>), dim3(blocks), dim3(threads), shm, s,  \
-                           (St *)W, g);                                                         \
+                           (St *)W, g.order, g);                                                \
     } while (0)
 #define PLL_TRID_S(M_, D_)                                                                      \
     switch (g.S) {                                                                              \
