@@ -258,12 +258,15 @@ __device__ __forceinline__ void stage_ct(V (&v)[E], int t, const V *__restrict__
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
         if constexpr (Ns > 1) {
+            // w^r from one table entry w: powers by products of depth <= 3
+            // (w^2 = w w, w^3 = w w^2, w^4 = w^2 w^2, ...), one load per butterfly
+            V wp[R];
+            wp[1] = tw[k * TWSTEP];
+            if (DIR > 0) wp[1].y = -wp[1].y;
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                V w = tw[r * k * TWSTEP];
-                if (DIR > 0) w.y = -w.y;
-                x[r] = cmul(x[r], w);
-            }
+            for (int r = 2; r < R; ++r) wp[r] = cmul(wp[r / 2], wp[r - r / 2]);
+#pragma unroll
+            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], wp[r]);
         }
         Dft<R, DIR>::run(x);
         if constexpr (LAST) {
