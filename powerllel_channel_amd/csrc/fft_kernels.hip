@@ -537,9 +537,17 @@ constexpr int max_threads_f()
 }
 
 // x forward: real rows -> packed half spectrum rows of W.
+// Recombination twiddle w^m, m = t + e T: w^t (one load per thread) times
+// w^(eT) (the same for the whole workgroup, so read through the scalar unit).
+template <typename V>
+__device__ __forceinline__ V tw_m(const V *__restrict__ tw, V wt, int e, int T)
+{
+    return e == 0 ? wt : cmul(wt, tw[e * T]);
+}
+
 template <class F, typename V, class Src>
 __global__ void __launch_bounds__(max_threads_f<F>())
-k_xfwd(Src src, V *__restrict__ W, XArgs<V> a)
+k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
 {
     using S = typename Sc<V>::T;
     extern __shared__ unsigned char lds_raw[];
@@ -563,6 +571,7 @@ k_xfwd(Src src, V *__restrict__ W, XArgs<V> a)
     for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
     __syncthreads();
     V *dst = W + row * L;
+    const V wt = twr[t];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
@@ -575,7 +584,7 @@ k_xfwd(Src src, V *__restrict__ W, XArgs<V> a)
             const V Ev = cscale(cadd(Z, Zc), S(0.5));
             const V D = csub(Z, Zc);
             const V O = mk<V>(S(0.5) * D.y, S(-0.5) * D.x);  // -i D / 2
-            X = cadd(Ev, cmul(a.tw[m], O));
+            X = cadd(Ev, cmul(tw_m(twr, wt, e, T), O));
         }
         if (valid) dst[m] = X;
     }
@@ -585,7 +594,7 @@ k_xfwd(Src src, V *__restrict__ W, XArgs<V> a)
 // with NORM, also the workgroup's max |value| and max |result|.
 template <class F, typename V, class Dst, bool NORM = false>
 __global__ void __launch_bounds__(max_threads_f<F>())
-k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
+k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
 {
     using S = typename Sc<V>::T;
     extern __shared__ unsigned char lds_raw[];
@@ -606,6 +615,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
 #pragma unroll
     for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
     __syncthreads();
+    const V wt = twr[t];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
@@ -616,7 +626,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, XArgs<V> a)
             const V Xc = cconj(ml[L - m]);
             const V Sm = cadd(X, Xc);
             const V D = csub(X, Xc);
-            const V Tt = cmul(cconj(a.tw[m]), D);
+            const V Tt = cmul(cconj(tw_m(twr, wt, e, T)), D);
             v[e] = mk<V>(Sm.x - Tt.y, Sm.y + Tt.x);  // S + i Tt
         }
     }
@@ -820,7 +830,7 @@ int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(V);
     if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     hipLaunchKernelGGL((k_xfwd<F, V, Src>), dim3((unsigned)blocks), dim3(threads), shm, s, src,
-                       (V *)W, a);
+                       (V *)W, a.tw, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -840,7 +850,7 @@ int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipS
         a.norm_part = norm + 2;
     }
     hipLaunchKernelGGL((k_xbwd<F, V, Dst, NORM>), dim3((unsigned)blocks), dim3(threads), shm, s,
-                       (const V *)W, dst, a);
+                       (const V *)W, dst, a.tw, a);
     if (hipGetLastError() != hipSuccess) return -3;
     if (NORM) {
         hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, s, (const double *)(norm + 2),
