@@ -874,7 +874,13 @@ int launch_y(bool fwd, const pll_ppe_dev *d, const void *tw, void *W, hipStream_
     const int threads = a.cols * a.P.T;
     const long long blocks = (long long)a.ntiles * d->nz;
     const size_t shm = (size_t)a.cols * a.pitch * sizeof(V);
-    if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    if (threads > max_threads_f<F>() || shm > 163840 || blocks > 0x7fffffffLL) return -1;
+    // long columns (ny = 1024 in FP64: 16 KB each) need more than the default
+    // 64 KB of LDS for a tile of 4 columns
+    const void *fn = fwd ? (const void *)k_yfwd<F, V> : (const void *)k_ybwd<F, V>;
+    if (shm > 65536 &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) != hipSuccess)
+        return -3;
     if (fwd)
         hipLaunchKernelGGL((k_yfwd<F, V>), dim3((unsigned)blocks), dim3(threads), shm, s, (V *)W, a);
     else

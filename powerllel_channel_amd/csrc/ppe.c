@@ -202,12 +202,14 @@ static int choose_launch(pll_ppe *h)
     if (rw * px->T > maxt || (size_t)rw * (px->L + 1) * 16 > 65536)
         return pll_set_error(PLL_EUNSUP, "nx=%d too large for the x transform kernel", h->nx);
     h->d.x_rows_per_wg = rw;
-    /* y kernels: adjacent columns per workgroup (>= 8 gives 128-byte rows) */
+    /* y kernels: adjacent columns per workgroup (8 gives 128-byte FP64 row
+     * segments; XCD-contiguous tile numbering lets narrower tiles share lines
+     * in L2), at most 80 KB of LDS so that two workgroups fit a CU */
     maxt = pll_fft_max_threads(py->E);
     int cols = 8;
     if (cols > h->L) cols = h->L;
-    while (cols > 1 && (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 65536)) cols--;
-    if (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 65536)
+    while (cols > 1 && (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 81920)) cols--;
+    if (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 81920)
         return pll_set_error(PLL_EUNSUP, "ny=%d too large for the y transform kernel", h->ny);
     h->d.y_cols_per_wg = cols;
     return PLL_OK;
@@ -224,9 +226,11 @@ static int part_fits(int nx, int nz, int m, int *M, int *P, int *S)
     if (m <= 0 || nz % m || !pll_trid_part_supported(m)) return 0;
     const int p = nz / m;
     if (p < 2 || p > 64 || (p & (p - 1))) return 0;
+    const char *ps = getenv("PLL_TRID_S"); /* tuning override of the tile width */
     for (size_t j = 0; j < sizeof Ss / sizeof Ss[0]; j++) {
         const int sc = Ss[j];
-        if (nx % sc || sc * p > 512 || (size_t)nz * sc * 16 + (size_t)nz * 24 + (size_t)p * 8 > 163840)
+        if (ps && atoi(ps) != sc) continue;
+        if (nx % sc || sc * p > 512 || (size_t)nz * sc * 8 + (size_t)nz * 24 + (size_t)p * 8 > 163840)
             continue;
         *M = m;
         *P = p;

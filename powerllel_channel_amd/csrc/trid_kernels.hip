@@ -203,12 +203,12 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
     const int tid = threadIdx.x;
     const long long plane = (long long)g.nx * g.ny;
     const int nthr = S * P;
-    // z coefficients staged in LDS after the tile and its pivot half: row
+    // z coefficients staged in LDS after the tile: row
     // k = p M + i of (a, c, bz) at cf[p CS + 3 i + {0, 1, 2}].  The odd pitch
     // CS = 3M + 1 puts the P lanes of a column in distinct banks, and a lane's
     // reads are at fixed offsets from one address.
     constexpr int CS = 3 * M + 1;
-    double *cf = tl + 2 * (size_t)nz * S;
+    double *cf = tl + (size_t)nz * S;
 
     for (int k = tid; k < nz; k += nthr) {
         const int o = (k / M) * CS + 3 * (k % M);
@@ -261,39 +261,82 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
     fetch(tile_next);
 
     if (q0 < g.qx && iy < g.iyx) {
-        // stored pivots of the tile's columns into the second half of LDS
-        double *pt = tl + nz * S;
+        // The reference's recurrences with the stored pivots (low modes, where
+        // the block elimination below loses accuracy):
+        //   forward  y_k = d_k - m_k y_{k-1},  m_k = a_k / piv_{k-1},
+        //   backward x_k = (y_k - c_k x_{k+1}) / piv_k,  x_{nz-1} from the top rule.
+        // Lane p of a column owns rows pM .. pM+M-1.  Each recurrence is affine
+        // in the value entering the lane, so a scan of the lanes' affine maps
+        // over the P lanes of the column gives every lane its entering value,
+        // and the lane then runs the recurrence over its rows as written.
+        const int k0 = p * M;
+        const int pb = k0 * S + (s ^ (p & (S - 1)));
+        const int lane = tid & 63;
         const long long pst = (long long)(g.L + 1) * (g.ny / 2 + 1);
-        const double *__restrict__ prow = g.piv + (long long)iy * (g.L + 1);
-#pragma unroll 1
-        for (int j = 0; j < M; ++j) {
-            const int e = tid + j * nthr;
-            const int k = e / S, s = e - k * S;
-            pt[swz(k, s, S, M)] = prow[k * pst + lam_ix(q0 + s, my, g.L, g.ny)];
+        const double *__restrict__ pc =
+            g.piv + (long long)iy * (g.L + 1) + lam_ix(q, my, g.L, g.ny);
+        const double *__restrict__ cl = cf + p * CS;
+        // the lane's rows stay in its LDS slots (registers hold the pivots)
+        double pv[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) pv[i] = pc[(long long)(k0 + i) * pst];
+        const double pleft = shfl(pv[M - 1], p > 0 ? lane - 1 : lane);  // pivot of row k0-1
+        double al = 0.0, be = 1.0;  // lane map: y(last row) = al + be * y(row k0-1)
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double m = (k0 + i > 0) ? cl[3 * i] / (i ? pv[i - 1] : pleft) : 0.0;
+            al = tl[pb + i * S] - m * al;
+            be = -m * be;
         }
-        __syncthreads();
-        if (tid < S) {
-            const int s = tid, q = q0 + s;
-            double prev = tl[swz(0, s, S, M)];
-#pragma unroll 4
-            for (int k = 1; k < nz; ++k) {
-                const int o = swz(k, s, S, M);
-                const double m = cf[(k / M) * CS + 3 * (k % M)] / pt[swz(k - 1, s, S, M)];
-                const double cur = tl[o] - m * prev;
-                tl[o] = cur;
-                prev = cur;
+        for (int t = 1; t < P; t *= 2) {
+            const bool h = p >= t;
+            const double al2 = shfl(al, h ? lane - t : lane), be2 = shfl(be, h ? lane - t : lane);
+            if (h) {
+                al = al + be * al2;
+                be = be * be2;
             }
-            const double bl = pt[swz(nz - 1, s, S, M)];
-            const bool pin = g.pin_gauge && my == 0 && q == 0;
-            prev = pin ? 0.0 : (bl != 0.0 ? prev / bl : 0.0);
-            tl[swz(nz - 1, s, S, M)] = prev;
-#pragma unroll 4
-            for (int k = nz - 2; k >= 0; --k) {
-                const int o = swz(k, s, S, M);
-                const double cur = (tl[o] - cf[(k / M) * CS + 3 * (k % M) + 1] * prev) / pt[o];
-                tl[o] = cur;
-                prev = cur;
+        }
+        double prev = shfl(al, p > 0 ? lane - 1 : lane);  // y(row k0-1)
+        if (p == 0) prev = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double m = (k0 + i > 0) ? cl[3 * i] / (i ? pv[i - 1] : pleft) : 0.0;
+            const double yi = tl[pb + i * S] - m * prev;
+            tl[pb + i * S] = yi;
+            prev = yi;
+        }
+        // top row: pinned gauge, y / piv, or 0 for an exactly zero pivot
+        const bool top = p == P - 1;
+        const bool pin = g.pin_gauge && my == 0 && q == 0;
+        const double bl = pv[M - 1];
+        const double xtop = pin ? 0.0 : (bl != 0.0 ? prev / bl : 0.0);
+        double ga = 0.0, de = 1.0;  // lane map: x(row k0) = ga + de * x(row k0+M)
+#pragma unroll
+        for (int i = M - 1; i >= 0; --i) {
+            if (top && i == M - 1) {
+                ga = xtop;
+                de = 0.0;
+            } else {
+                const double c = cl[3 * i + 1];
+                ga = (tl[pb + i * S] - c * ga) / pv[i];
+                de = -(c * de) / pv[i];
             }
+        }
+        for (int t = 1; t < P; t *= 2) {
+            const bool h = p + t < P;
+            const double ga2 = shfl(ga, h ? lane + t : lane), de2 = shfl(de, h ? lane + t : lane);
+            if (h) {
+                ga = ga + de * ga2;
+                de = de * de2;
+            }
+        }
+        double next = shfl(ga, top ? lane : lane + 1);  // x(row k0+M)
+#pragma unroll
+        for (int i = M - 1; i >= 0; --i) {
+            const double x =
+                (top && i == M - 1) ? xtop : (tl[pb + i * S] - cl[3 * i + 1] * next) / pv[i];
+            tl[pb + i * S] = x;
+            next = x;
         }
     } else {
         const int k0 = p * M;
@@ -475,8 +518,8 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         g.qx = d->tp.qx;
         g.iyx = d->tp.iyx;
         const int threads = g.S * g.P;
-        // values + pivots + coefficients (P rows of 3M + 1)
-        const size_t shm = sizeof(double) * (2 * (size_t)d->nz * g.S + 3 * (size_t)d->nz + g.P);
+        // values + coefficients (P rows of 3M + 1)
+        const size_t shm = sizeof(double) * ((size_t)d->nz * g.S + 3 * (size_t)d->nz + g.P);
         if (threads > 512 || shm > 163840) return -1;
         switch (d->tp.M) {
 #define PLL_TRID_LAUNCH(M_, S_, D_)                                                             \
