@@ -403,7 +403,8 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
              * round-robin, so hand each XCD a contiguous run of this list:
              * neighbouring tiles (sharing 128-byte lines in FP32 storage)
              * then meet in one L2 */
-            if (ntiles % 8 == 0) {
+            const char *px = getenv("PLL_TRID_XCD"); /* tuning: 0 keeps the plain order */
+            if (ntiles % 8 == 0 && !(px && atoi(px) == 0)) {
                 int *perm = (int *)malloc(sizeof(int) * (size_t)ntiles);
                 if (!perm) {
                     free(order);
