@@ -11,7 +11,6 @@
 //   calcMaxDiv          src/mod_utils.f90:257-318
 // Fields are the reference's haloed arrays f(0:nx+1,0:ny+1,0:nz+1), x fastest.
 #include <hip/hip_runtime.h>
-#include <stdlib.h>
 
 #include <type_traits>
 
@@ -93,17 +92,15 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
     const long long sy = g.sy, sz = g.sz;
     const double *__restrict__ Q[3] = {qu, qv, qw};
-    // this thread's share of a plane tile: element q = tid + r * 256.  All
-    // global loads are unconditional, from addresses clamped into the array
-    // (values outside the domain or past the tile are never read by a valid
-    // cell): loads issued under a branch make the compiler wait for every
-    // outstanding load at the join, prefetch included.
+    // this thread's share of a plane tile: element q = tid + r * 256
     long long go[NLD];
+    bool gk[NLD];
 #pragma unroll
     for (int r = 0; r < NLD; ++r) {
-        const int q = min(tid + r * RK_TX * RK_TY, LPL - 1);
+        const int q = tid + r * RK_TX * RK_TY;
         const int row = q / LTX, col = q - row * LTX;
-        const int gi = min(i0 + col, g.nx + 1), gj = min(j0 + row, g.ny + 1);
+        const int gi = i0 + col, gj = j0 + row;
+        gk[r] = q < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
         go[r] = gi + sy * gj;
     }
     double pre[3][NLD];
@@ -111,7 +108,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
 #pragma unroll
         for (int f = 0; f < 3; ++f)
 #pragma unroll
-            for (int r = 0; r < NLD; ++r) pre[f][r] = Q[f][go[r] + sz * kp];
+            for (int r = 0; r < NLD; ++r) pre[f][r] = gk[r] ? Q[f][go[r] + sz * kp] : 0.0;
     };
     auto put = [&](int b) {
 #pragma unroll
@@ -126,38 +123,22 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     put(0);
     fetch(k0 + 1);
     put(1);
-    // this cell, or the tile's first cell for lanes outside the domain
-    long long e = (valid ? i + sy * j : (i0 + 1) + sy * (j0 + 1)) + sz * k0;
-    double u00m = qu[e - sz], v00m = qv[e - sz], w00m = qw[e - sz];
-    double wp0m = qw[e - sz + 1], w0pm = qw[e - sz + sy];
-    // old values (RK2) and body force of the current plane, loaded one step
-    // ahead: the wait for them then never includes the plane prefetch
-    double ou0 = 0.0, ov0 = 0.0, ow0 = 0.0, bu = 0.0, bv = 0.0, bw = 0.0;
-    auto step_loads = [&](long long ee, double &a0, double &a1, double &a2, double &b0,
-                          double &b1, double &b2) {
-        if (SECOND) {
-            a0 = ou[ee];
-            a1 = ov[ee];
-            a2 = ow[ee];
-        }
-        if (BODY) {
-            b0 = fb.u[ee];
-            b1 = fb.v[ee];
-            b2 = fb.w[ee];
-        }
-    };
-    step_loads(e, ou0, ov0, ow0, bu, bv, bw);
+    long long e = i + sy * j + sz * k0;
+    double u00m = 0.0, v00m = 0.0, w00m = 0.0, wp0m = 0.0, w0pm = 0.0;
+    if (valid) {
+        u00m = qu[e - sz];
+        v00m = qv[e - sz];
+        w00m = qw[e - sz];
+        wp0m = qw[e - sz + 1];
+        w0pm = qw[e - sz + sy];
+    }
     __syncthreads();
     double su = 0.0, sv = 0.0;
     const int c0 = (ly + 1) * LTX + lx + 1;  // this cell in a tile
     for (int k = k0; k <= k1; ++k, e += sz) {
         const int b = (k - k0) & 1;
         const bool more = k + 2 <= k1 + 1;
-        // prefetch of plane k+2 (the last step re-reads plane k1+1), then the
-        // next step's loads (the last step re-reads its own cell)
-        fetch(more ? k + 2 : k1 + 1);
-        double ou1 = 0.0, ov1 = 0.0, ow1 = 0.0, bu1 = 0.0, bv1 = 0.0, bw1 = 0.0;
-        step_loads(k < k1 ? e + sz : e, ou1, ov1, ow1, bu1, bv1, bw1);
+        if (more) fetch(k + 2);
         if (valid) {
             const double *cu = tl[b][0], *cv = tl[b][1], *cw = tl[b][2];
             const double *nu = tl[b ^ 1][0], *nv = tl[b ^ 1][1], *nw = tl[b ^ 1][2];
@@ -174,16 +155,16 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
             double fu, fv, fw;
             mom_rhs(g, n, k, fu, fv, fw);
             if (BODY) {
-                fu = fu + bu;
-                fv = fv + bv;
-                fw = fw + bw;
+                fu = fu + fb.u[e];
+                fv = fv + fb.v[e];
+                fw = fw + fb.w[e];
             }
             if (SECOND) {
-                const double un = (n.u000 + g.dt * fu + ou0) * 0.5;
-                const double vn = (n.v000 + g.dt * fv + ov0) * 0.5;
+                const double un = (n.u000 + g.dt * fu + ou[e]) * 0.5;
+                const double vn = (n.v000 + g.dt * fv + ov[e]) * 0.5;
                 ou[e] = un;
                 ov[e] = vn;
-                ow[e] = (n.w000 + g.dt * fw + ow0) * 0.5;
+                ow[e] = (n.w000 + g.dt * fw + ow[e]) * 0.5;
                 if (MEAN) {
                     const double wk = g.dzflzi[k];
                     if (fmask & 1) su = su + un * wk;
@@ -200,8 +181,6 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
             wp0m = n.wp00;
             w0pm = n.w0p0;
         }
-        ou0 = ou1; ov0 = ov1; ow0 = ow1;
-        bu = bu1; bv = bv1; bw = bw1;
         __syncthreads();  // plane k's buffer is free
         if (more) put(b);  // it becomes plane k+2
         __syncthreads();
@@ -251,28 +230,17 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_correct(G g, Tiles t, const do
         if (fmask & 1) fu = vfu - mean[0];
         if (fmask & 2) fv = vfv - mean[1];
     }
-    // the next plane's values are loaded before this plane is computed
-    double pc = p[e], pz = p[e + sz], pxp = p[e + 1], pyp = p[e + sy];
-    double uc = u[e], vc = v[e], wc = w[e];
-#pragma unroll 2
+    double pc = p[e];
     for (int k = k0; k <= k1; ++k, e += sz) {
-        const long long en = k < k1 ? e + sz : e;
-        const double pz1 = p[en + sz], px1 = p[en + 1], py1 = p[en + sy];
-        const double u1 = u[en], v1 = v[en], w1 = w[en];
-        double un = uc - (pxp - pc) * dtdxi;
-        double vn = vc - (pyp - pc) * dtdyi;
+        const double pz = p[e + sz];
+        double un = u[e] - (p[e + 1] - pc) * dtdxi;
+        double vn = v[e] - (p[e + sy] - pc) * dtdyi;
         if (FORCE && (fmask & 1)) un = un + fu;
         if (FORCE && (fmask & 2)) vn = vn + fv;
         u[e] = un;
         v[e] = vn;
-        w[e] = wc - (pz - pc) * g.dt * g.dzc_inv[k];
+        w[e] = w[e] - (pz - pc) * g.dt * g.dzc_inv[k];
         pc = pz;
-        pz = pz1;
-        pxp = px1;
-        pyp = py1;
-        uc = u1;
-        vc = v1;
-        wc = w1;
     }
 }
 
@@ -535,15 +503,10 @@ __global__ void __launch_bounds__(256) k_nan_part(const double *__restrict__ q, 
 
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -3; }
 
-// kc: planes per workgroup (PLL_RK_KC overrides the default for tuning)
-Tiles rk_tiles(const G &g, int kc = 0)
+Tiles rk_tiles(const G &g)
 {
     Tiles t;
-    if (kc <= 0) {
-        const char *e = getenv("PLL_RK_KC");
-        kc = e && atoi(e) > 0 ? atoi(e) : 64;
-    }
-    t.kc = kc;
+    t.kc = 64;
     t.ntx = (g.nx + RK_TX - 1) / RK_TX;
     t.nty = (g.ny + RK_TY - 1) / RK_TY;
     t.nkc = (g.nz + t.kc - 1) / t.kc;
@@ -621,8 +584,7 @@ extern "C" int pll_launch_correct_vel(const pll_grid_dev *gd, const double *p, d
                                       const double *mean, hipStream_t s)
 {
     const G g = make_g(gd);
-    const char *ekc = getenv("PLL_CORRECT_KC");
-    const Tiles t = rk_tiles(g, ekc && atoi(ekc) > 0 ? atoi(ekc) : 8);
+    const Tiles t = rk_tiles(g);
     const double dtdxi = g.dt * g.dx_inv, dtdyi = g.dt * g.dy_inv;  // mod_calcVel.f90:436-437
     const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc));
     const bool force = mean && fmask;
