@@ -256,3 +256,40 @@ def test_monitors_vs_oracle(orc, cuda):
         assert ch.check_nan(bad)
         passed, rep = ch.check(bad, dv, dw, cfl_limit=1e9, div_limit=1e9)
         assert not passed and rep["nan"]
+
+
+def test_refined_steps_vs_oracle(orc, cuda):
+    """RK2 steps with the PPE in the refined form, to a 1e-11 pressure
+    tolerance, against the oracle's FP64 steps (the reference algorithm):
+    the precision form's velocity, pressure and divergence errors."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, mean_removed, poiseuille_fields
+    nx, ny, nz, nu, dt, nsteps = 64, 48, 32, 1 / 5600, 1e-3, 20
+    print(f"[refined steps] {nx}x{ny}x{nz}, {nsteps} steps, tolerance 1e-11", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 4.0)
+    G = orc.OracleGrid(nx, ny, nz, LX, LY, LZ, g.dzf, nu, dt)
+    u, v, w = poiseuille_fields(g, 6)
+    orc.bc_vel(G, u, v, w, 0.0)
+    u1, v1, w1, p = [np.zeros(g.shape) for _ in range(4)]
+    du, dv, dw, du1, dv1, dw1, dp = _dev(torch, cuda, u, v, w, u1, v1, w1, p)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, nu, dt)
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    s.set_refinement_tol(1e-11, 8)
+    ppe = orc.OraclePPE(nx, ny, nz, g.dx, g.dy, g.dzf, pin_gauge=True)
+    worst_div = 0.0
+    for _ in range(nsteps):
+        ch.step(s, du, dv, dw, du1, dv1, dw1, dp)
+        orc.step(G, ppe, u, v, w, u1, v1, w1, p)
+        torch.cuda.synchronize()
+        worst_div = max(worst_div, ch.max_div(du, dv, dw)[0])
+    sweeps, est = s.last_refinement()
+    ev = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(_host(du, dv, dw), (u, v, w)))
+    gp = dp.cpu().numpy()
+    ep = float(np.abs(mean_removed(g, gp) - mean_removed(g, p)).max() / np.abs(mean_removed(g, p)).max())
+    print(f"  last solve {sweeps} sweeps (estimate {est:.1e}); velocity {ev:.2e}, "
+          f"pressure {ep:.2e}, max|div u| {worst_div:.2e}")
+    assert ev <= 1e-10
+    assert ep <= 1e-9
+    assert worst_div < 1e-9  # the reference's run-time limit, examples/template/param.in:33
