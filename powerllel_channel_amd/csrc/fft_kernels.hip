@@ -247,7 +247,7 @@ __device__ __forceinline__ void fft_run(V (&v)[E], int t, const pll_fftplan &P,
 // the butterfly indices (b mod Ns, output positions, twiddle offsets) fold
 // into shifts and small constants and nothing is spilled between radices.
 template <int R, int E, int DIR, int T, int Ns, int TWSTEP, bool LAST, typename V>
-__device__ __forceinline__ void stage_ct(V (&v)[E], int t, const V *__restrict__ tw, V *lds)
+__device__ __forceinline__ void stage_ct(V (&v)[E], int t, const V *__restrict__ tw)
 {
     constexpr int Q = E / R;
 #pragma unroll
@@ -269,60 +269,130 @@ __device__ __forceinline__ void stage_ct(V (&v)[E], int t, const V *__restrict__
             for (int r = 1; r < R; ++r) x[r] = cmul(x[r], wp[r]);
         }
         Dft<R, DIR>::run(x);
-        if constexpr (LAST) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
-        } else {
-            const int base = (b - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) lds[base + r * Ns] = x[r];
-        }
+        for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
     }
 }
 
+// Between stages, output r of butterfly q (in v[q + r*Q]) moves to position
+// (b - k) R + k + r Ns of the row (b = t + qT, k = b mod Ns), and element
+// t + eT of the next stage comes back.  The exchange goes through LDS one
+// real component at a time, so a row needs L scalars of LDS, not L complex
+// values: twice the workgroups fit a CU.
 template <typename V, int L, int E, int DIR, int TWSCALE, int Ns, int R, int... Rest>
-__device__ __forceinline__ void stages_ct(V (&v)[E], int t, const V *__restrict__ tw, V *lds)
+__device__ __forceinline__ void stages_ct(V (&v)[E], int t, const V *__restrict__ tw,
+                                          typename Sc<V>::T *ls)
 {
     static_assert(E % R == 0, "radix must divide E");
-    constexpr int T = L / E;
+    constexpr int T = L / E, Q = E / R;
     constexpr bool LAST = sizeof...(Rest) == 0;
     constexpr int TWSTEP = TWSCALE * (L / (Ns * R));
-    stage_ct<R, E, DIR, T, Ns, TWSTEP, LAST>(v, t, tw, lds);
+    stage_ct<R, E, DIR, T, Ns, TWSTEP, LAST>(v, t, tw);
     if constexpr (!LAST) {
+        auto put = [&](bool imag) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int b = t + q * T;
+                const int k = b % Ns;
+                const int base = (b - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    ls[base + r * Ns] = imag ? v[q + r * Q].y : v[q + r * Q].x;
+            }
+        };
+        put(false);
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = lds[t + e * T];
+        for (int e = 0; e < E; ++e) v[e].x = ls[t + e * T];
         __syncthreads();
-        stages_ct<V, L, E, DIR, TWSCALE, Ns * R, Rest...>(v, t, tw, lds);
+        put(true);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e].y = ls[t + e * T];
+        __syncthreads();
+        stages_ct<V, L, E, DIR, TWSCALE, Ns * R, Rest...>(v, t, tw, ls);
     }
 }
 
 // FFT policies used by the kernels: a compile-time plan, or the runtime plan.
+// kSplit: the plan exchanges through LDS one real component at a time (a row
+// segment holds L+1 scalars); otherwise complex values (L+1 of them).
 template <int L_, int E_, int... Rs>
 struct CtFFT {
     static constexpr int E = E_;
+    static constexpr bool kSplit = true;
     static __device__ __forceinline__ int threads(const pll_fftplan &) { return L_ / E_; }
     static __device__ __forceinline__ int length(const pll_fftplan &) { return L_; }
     template <int DIR, int TWSCALE, typename V>
     static __device__ __forceinline__ void run(V (&v)[E_], int t, const pll_fftplan &,
-                                               const V *__restrict__ tw, V *lds)
+                                               const V *__restrict__ tw, void *lds)
     {
-        stages_ct<V, L_, E_, DIR, TWSCALE, 1, Rs...>(v, t, tw, lds);
+        stages_ct<V, L_, E_, DIR, TWSCALE, 1, Rs...>(v, t, tw,
+                                                     reinterpret_cast<typename Sc<V>::T *>(lds));
     }
 };
 
 template <int E_>
 struct RtFFT {
     static constexpr int E = E_;
+    static constexpr bool kSplit = false;
     static __device__ __forceinline__ int threads(const pll_fftplan &P) { return P.T; }
     static __device__ __forceinline__ int length(const pll_fftplan &P) { return P.L; }
     template <int DIR, int TWSCALE, typename V>
     static __device__ __forceinline__ void run(V (&v)[E_], int t, const pll_fftplan &P,
-                                               const V *__restrict__ tw, V *lds)
+                                               const V *__restrict__ tw, void *lds)
     {
-        fft_run<E_, DIR>(v, t, P, tw, TWSCALE, lds);
+        fft_run<E_, DIR>(v, t, P, tw, TWSCALE, reinterpret_cast<V *>(lds));
     }
 };
+
+// Bytes of LDS per element of a row segment for plan F.
+template <class F, typename V>
+constexpr size_t lds_elem()
+{
+    return F::kSplit ? sizeof(typename Sc<V>::T) : sizeof(V);
+}
+
+// mv[e] := value at position n - m of the row (m = t + eT; position 0 for
+// m = 0), where v[e] holds the value at m: the mirror the real-to-complex
+// packing needs.  Every thread of the workgroup calls it (barriers); the
+// caller synchronises before the row segment is written again.
+template <class F, typename V, int E>
+__device__ __forceinline__ void mirror(const V (&v)[E], int t, int T, int n, unsigned char *row,
+                                       V (&mv)[E])
+{
+    using S = typename Sc<V>::T;
+    if constexpr (F::kSplit) {
+        S *ls = reinterpret_cast<S *>(row);
+#pragma unroll
+        for (int e = 0; e < E; ++e) ls[t + e * T] = v[e].x;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            mv[e].x = ls[m ? n - m : 0];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) ls[t + e * T] = v[e].y;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            mv[e].y = ls[m ? n - m : 0];
+        }
+    } else {
+        V *lv = reinterpret_cast<V *>(row);
+#pragma unroll
+        for (int e = 0; e < E; ++e) lv[t + e * T] = v[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            mv[e] = lv[m ? n - m : 0];
+        }
+    }
+}
 
 // ---------------------------------------------------------------- x I/O
 // Sources of the x forward transform: real row (j,k) of nx values.
@@ -551,13 +621,12 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
 {
     using S = typename Sc<V>::T;
     extern __shared__ unsigned char lds_raw[];
-    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
     const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
-    V *ml = lds + rl * a.pitch;
+    unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
     V v[E];
 #pragma unroll
@@ -567,9 +636,8 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
     }
     F::template run<-1, 2>(v, t, a.P, a.tw, ml);
     __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
-    __syncthreads();
+    V zm[E];
+    mirror<F, V, E>(v, t, T, L, ml, zm);
     V *dst = W + row * L;
     const V wt = twr[t];
 #pragma unroll
@@ -580,7 +648,7 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
         if (m == 0) {
             X = mk<V>(Z.x + Z.y, Z.x - Z.y);  // (X_0, X_L)
         } else {
-            const V Zc = cconj(ml[L - m]);
+            const V Zc = cconj(zm[e]);
             const V Ev = cscale(cadd(Z, Zc), S(0.5));
             const V D = csub(Z, Zc);
             const V O = mk<V>(S(0.5) * D.y, S(-0.5) * D.x);  // -i D / 2
@@ -598,13 +666,12 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
 {
     using S = typename Sc<V>::T;
     extern __shared__ unsigned char lds_raw[];
-    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
     const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
-    V *ml = lds + rl * a.pitch;
+    unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
     const V *src = W + row * L;
     V v[E];
@@ -612,18 +679,17 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
     // LDS write directly is otherwise waited for one at a time)
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? src[t + e * T] : mk<V>(S(0), S(0));
-#pragma unroll
-    for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
-    __syncthreads();
+    V xm[E];
+    mirror<F, V, E>(v, t, T, L, ml, xm);
     const V wt = twr[t];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * T;
-        const V X = ml[m];
+        const V X = v[e];
         if (m == 0) {
             v[e] = mk<V>(X.x + X.y, X.x - X.y);
         } else {
-            const V Xc = cconj(ml[L - m]);
+            const V Xc = cconj(xm[e]);
             const V Sm = cadd(X, Xc);
             const V D = csub(X, Xc);
             const V Tt = cmul(cconj(tw_m(twr, wt, e, T)), D);
@@ -709,7 +775,6 @@ k_yfwd(V *__restrict__ W, YArgs<V> a)
 {
     using S = typename Sc<V>::T;
     extern __shared__ unsigned char lds_raw[];
-    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
     const unsigned b = xcd_block();
@@ -719,7 +784,7 @@ k_yfwd(V *__restrict__ W, YArgs<V> a)
     const int mx = tile * a.cols + c;
     const bool valid = mx < L;
     V *col = W + k * (long long)ny * L + (valid ? mx : 0);
-    V *ml = lds + c * a.pitch;
+    unsigned char *ml = lds_raw + (size_t)c * a.pitch * lds_elem<F, V>();
     V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -731,18 +796,17 @@ k_yfwd(V *__restrict__ W, YArgs<V> a)
         // column 0 holds DFT_y(X_0 + i X_L): split into A = DFT_y(X_0) and
         // B = DFT_y(X_L), keeping one half of each Hermitian sequence.
         __syncthreads();
-#pragma unroll
-        for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
-        __syncthreads();
+        V vm[E];
+        mirror<F, V, E>(v, t, T, ny, ml, vm);
         if (c == 0) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int my = t + e * T;
                 if (my == 0 || 2 * my == ny) continue;  // (A, B) = (Re C, Im C)
                 if (2 * my < ny) {
-                    v[e] = cscale(cadd(v[e], cconj(ml[ny - my])), S(0.5));  // A_my
+                    v[e] = cscale(cadd(v[e], cconj(vm[e])), S(0.5));  // A_my
                 } else {
-                    const V D = csub(ml[ny - my], cconj(v[e]));
+                    const V D = csub(vm[e], cconj(v[e]));
                     v[e] = mk<V>(S(0.5) * D.y, S(-0.5) * D.x);  // B_{ny-my}
                 }
             }
@@ -760,7 +824,6 @@ k_ybwd(V *__restrict__ W, YArgs<V> a)
 {
     using S = typename Sc<V>::T;
     extern __shared__ unsigned char lds_raw[];
-    V *lds = reinterpret_cast<V *>(lds_raw);
     constexpr int E = F::E;
     const int T = F::threads(a.P), ny = F::length(a.P), L = a.L;
     const unsigned b = xcd_block();
@@ -770,7 +833,7 @@ k_ybwd(V *__restrict__ W, YArgs<V> a)
     const int mx = tile * a.cols + c;
     const bool valid = mx < L;
     V *col = W + k * (long long)ny * L + (valid ? mx : 0);
-    V *ml = lds + c * a.pitch;
+    unsigned char *ml = lds_raw + (size_t)c * a.pitch * lds_elem<F, V>();
     V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -779,19 +842,18 @@ k_ybwd(V *__restrict__ W, YArgs<V> a)
     }
     if (tile == 0) {
         // rebuild C = A + i B over all my from the packed halves
-#pragma unroll
-        for (int e = 0; e < E; ++e) ml[t + e * T] = v[e];
-        __syncthreads();
+        V vm[E];
+        mirror<F, V, E>(v, t, T, ny, ml, vm);
         if (c == 0) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int my = t + e * T;
                 if (my == 0 || 2 * my == ny) continue;
                 if (2 * my < ny) {
-                    const V B = ml[ny - my];
+                    const V B = vm[e];
                     v[e] = mk<V>(v[e].x - B.y, v[e].y + B.x);
                 } else {
-                    const V A = cconj(ml[ny - my]);
+                    const V A = cconj(vm[e]);
                     const V B = cconj(v[e]);
                     v[e] = mk<V>(A.x - B.y, A.y + B.x);
                 }
@@ -827,7 +889,7 @@ int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_
     const XArgs<V> a = x_args<V>(d, tw);
     const int threads = a.rows_per_wg * a.P.T;
     const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
-    const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(V);
+    const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
     if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     hipLaunchKernelGGL((k_xfwd<F, V, Src>), dim3((unsigned)blocks), dim3(threads), shm, s, src,
                        (V *)W, a.tw, a);
@@ -841,7 +903,7 @@ int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipS
     XArgs<V> a = x_args<V>(d, tw);
     const int threads = a.rows_per_wg * a.P.T;
     const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
-    const size_t shm = (size_t)a.rows_per_wg * a.pitch * sizeof(V);
+    const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
     if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     if (NORM) {
         // norm[0..1]: result; norm[2 ..): partials, room for ny*nz workgroups
@@ -873,7 +935,7 @@ int launch_y(bool fwd, const pll_ppe_dev *d, const void *tw, void *W, hipStream_
     a.pitch = d->ny + 1;
     const int threads = a.cols * a.P.T;
     const long long blocks = (long long)a.ntiles * d->nz;
-    const size_t shm = (size_t)a.cols * a.pitch * sizeof(V);
+    const size_t shm = (size_t)a.cols * a.pitch * lds_elem<F, V>();
     if (threads > max_threads_f<F>() || shm > 163840 || blocks > 0x7fffffffLL) return -1;
     // long columns (ny = 1024 in FP64: 16 KB each) need more than the default
     // 64 KB of LDS for a tile of 4 columns
