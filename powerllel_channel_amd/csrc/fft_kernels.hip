@@ -537,19 +537,6 @@ __device__ __forceinline__ double nmax(double m, double x)
 {
     return (x > m || x != x) ? x : m;  // NaN wins
 }
-struct DstHaloSetN {  // haloed FP64 interior := value
-    double *p;
-    long long off1, off2;
-    __device__ __forceinline__ double store_n(long long, long long j, long long k, int m, int,
-                                              double a, double b) const
-    {
-        d2u pr;
-        pr.x = a;
-        pr.y = b;
-        *reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m) = pr;
-        return nmax(fabs(a), fabs(b));
-    }
-};
 struct DstHaloAddN {  // haloed FP64 interior := (interior - eps) + value
     double *p;
     long long off1, off2;
@@ -1114,19 +1101,13 @@ extern "C" int pll_launch_xfwd32_copy(const pll_ppe_dev *d, const double *f_halo
     });
 }
 
-extern "C" int pll_launch_xbwd32_norm(const pll_ppe_dev *d, int add, const void *W32,
-                                      double *p_halo, const double *eps, double *norm,
-                                      hipStream_t s)
+extern "C" int pll_launch_xbwd32_norm(const pll_ppe_dev *d, const void *W32, double *p_halo,
+                                      const double *eps, double *norm, hipStream_t s)
 {
     const long long off1 = (long long)(d->nx + 2) * (d->ny + 2), off2 = d->nx + 2;
     return dispatch_plan(d->px, [&](auto f) {
-        using F = decltype(f);
-        if (add)
-            return launch_xb<F, float2, DstHaloAddN, true>(d, d->tw_x32, W32,
-                                                            DstHaloAddN{p_halo, off1, off2, eps},
-                                                            s, norm);
-        return launch_xb<F, float2, DstHaloSetN, true>(d, d->tw_x32, W32,
-                                                        DstHaloSetN{p_halo, off1, off2}, s, norm);
+        return launch_xb<decltype(f), float2, DstHaloAddN, true>(
+            d, d->tw_x32, W32, DstHaloAddN{p_halo, off1, off2, eps}, s, norm);
     });
 }
 

@@ -93,11 +93,11 @@ int pll_launch_xbwd32(const pll_ppe_dev *d, int dst_kind, const void *W32, doubl
 /* Refinement with a tolerance (the solution lives in the caller's haloed
  * array, the right-hand side is kept in a dense FP64 copy F):
  * x forward of the haloed f that also writes F; x backward into the haloed
- * p (add = 0: p := c; add = 1: p := (p - *eps) + c) reporting, in norm[0]
- * and norm[1], max |c| and max |p| (norm holds 2 + 2*ny*nz doubles). */
+ * p, p := (p - *eps) + c, reporting in norm[0] and norm[1] max |c| and
+ * max |p| (norm holds 2 + 2*ny*nz doubles). */
 int pll_launch_xfwd32_copy(const pll_ppe_dev *d, const double *f_halo, double *F, void *W32,
                            hipStream_t s);
-int pll_launch_xbwd32_norm(const pll_ppe_dev *d, int add, const void *W32, double *p_halo,
+int pll_launch_xbwd32_norm(const pll_ppe_dev *d, const void *W32, double *p_halo,
                            const double *eps, double *norm, hipStream_t s);
 /* residual with a dense f and a haloed p (see pll_launch_resid) */
 int pll_launch_resid_hp(const pll_ppe_dev *d, const double *F, const double *p_halo, float *r,

@@ -703,7 +703,7 @@ int pll_ppe_set_refinement(pll_ppe *h, int sweeps)
  * of the first two corrections 5e-7, error after one sweep 3e-11); from the
  * second sweep on the corrections carry those modes and rho_k tracks the
  * contraction.  Each sweep reads the two norms back: one stream
- * synchronisation per sweep. */
+ * synchronisation per sweep (none for the first FP32 solve). */
 static int ensure_norm(pll_ppe *h)
 {
     if (h->nrm) return PLL_OK;
@@ -735,12 +735,12 @@ int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_swe
     if ((rc = launch_rc(pll_launch_xfwd32_copy(&h->d, d_var, F, h->W32, s), "x forward (fp32)")))
         return rc;
     if ((rc = spectral32(h, s))) return rc;
-    if ((rc = launch_rc(pll_launch_xbwd32_norm(&h->d, 0, h->W32, d_var, NULL, nrm, s),
+    /* the first solve's norms enter no estimate: no reduction, no read-back */
+    if ((rc = launch_rc(pll_launch_xbwd32(&h->d, PLL_DST_HALO_SET, h->W32, d_var, NULL, NULL, s),
                         "x backward (fp32)")))
         return rc;
     double n[2];
-    if ((rc = read_norms(h, s, n))) return rc;
-    double cprev = n[0], est = 1.0;
+    double cprev = 0.0, est = 1.0;
     double *gauge = h->d.pin_gauge ? (double *)h->gauge : NULL;
     const double *eps = gauge ? gauge + pll_gauge_parts(&h->d) : NULL;
     int it;
@@ -752,7 +752,7 @@ int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_swe
                             "x forward (fp32)")))
             return rc;
         if ((rc = spectral32(h, s))) return rc;
-        if ((rc = launch_rc(pll_launch_xbwd32_norm(&h->d, 1, h->W32, d_var, eps, nrm, s),
+        if ((rc = launch_rc(pll_launch_xbwd32_norm(&h->d, h->W32, d_var, eps, nrm, s),
                             "x backward (fp32)")))
             return rc;
         if ((rc = read_norms(h, s, n))) return rc;
