@@ -97,7 +97,10 @@ def bench_ppe(torch, dev, steps, warmup):
 # FP64 8); each sweep 72 (residual reads f 8 + P 8 writes r 4; x_fwd 8; y, z, y
 # 24; x_bwd reads W32 4 + P 8 writes 8).
 def refined_bytes_per_cell(k):
-    return 48 + 72 * k
+    """FP32 solve 48 B/cell (read f64, write p64, FP32 passes); each sweep
+    64 B/cell: residual formed in the x transform (read f, p; write 4),
+    y/z/y 24, x backward with the update (read 4, read and write p)."""
+    return 48 + 64 * k
 
 
 def bench_refined(torch, dev, steps, warmup, sweeps=(0, 1, 2)):

@@ -418,6 +418,51 @@ struct SrcHalo64Copy {  // SrcHalo64, also copying the FP64 values to a dense ar
         return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
     }
 };
+// Residual r = f - L_h p of a refinement sweep, formed in the load of the x
+// transform: the arithmetic of k_resid (neighbour differences, row sums),
+// rounded to the transform's precision.  f and p are each haloed or dense
+// (element (i,j,k) at base + k s1 + j s2 + i).
+struct SrcResid {
+    const double *f, *P;
+    long long fb, f1, f2, pb, p1, p2;
+    const double *a, *c, *d;
+    double idx2, idy2;
+    int ny, nz;
+    template <typename V>
+    __device__ __forceinline__ V load(long long, long long j, long long k, int m, int nx) const
+    {
+        const int i0 = 2 * m, i1 = i0 + 1;
+        const int im = i0 == 0 ? nx - 1 : i0 - 1, ip = i1 + 1 == nx ? 0 : i1 + 1;
+        const long long jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+        const double *__restrict__ row = P + pb + k * p1 + j * p2;
+        const double *__restrict__ rjp = P + pb + k * p1 + jp * p2;
+        const double *__restrict__ rjm = P + pb + k * p1 + jm * p2;
+        const double c0 = row[i0], c1 = row[i1];
+        const double xm = row[im], xp = row[ip];
+        const double yp0 = rjp[i0], yp1 = rjp[i1], ym0 = rjm[i0], ym1 = rjm[i1];
+        double zm0 = 0.0, zm1 = 0.0, zp0 = 0.0, zp1 = 0.0;
+        if (k > 0) {
+            zm0 = row[i0 - p1];
+            zm1 = row[i1 - p1];
+        }
+        if (k + 1 < nz) {
+            zp0 = row[i0 + p1];
+            zp1 = row[i1 + p1];
+        }
+        const double ak = a[k], ck = c[k], dk = d[k];
+        const double *__restrict__ fr = f + fb + k * f1 + j * f2;
+        const double f0 = fr[i0], fv1 = fr[i1];
+        const double lx0 = ((c1 - c0) + (xm - c0)) * idx2;
+        const double lx1 = ((xp - c1) + (c0 - c1)) * idx2;
+        const double ly0 = ((yp0 - c0) + (ym0 - c0)) * idy2;
+        const double ly1 = ((yp1 - c1) + (ym1 - c1)) * idy2;
+        const double lz0 = ak * (zm0 - c0) + ck * (zp0 - c0) + dk * c0;
+        const double lz1 = ak * (zm1 - c1) + ck * (zp1 - c1) + dk * c1;
+        const double r0 = f0 - (lx0 + ly0 + lz0), r1 = fv1 - (lx1 + ly1 + lz1);
+        return mk<V>((typename Sc<V>::T)(float)r0, (typename Sc<V>::T)(float)r1);
+    }
+};
+
 struct SrcDense32 {  // dense FP32 rows (the residual of a refinement sweep)
     const float *r;
     template <typename V>
@@ -1098,6 +1143,32 @@ extern "C" int pll_launch_xfwd32_copy(const pll_ppe_dev *d, const double *f_halo
     const SrcHalo64Copy in{f_halo, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2, F};
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xf<decltype(f), float2>(d, d->tw_x32, in, W32, s);
+    });
+}
+
+extern "C" int pll_launch_xfwd32_resid(const pll_ppe_dev *d, const double *f, int f_halo,
+                                       const double *P, int p_halo, void *W32, hipStream_t s)
+{
+    const long long h1 = (long long)(d->nx + 2) * (d->ny + 2), h2 = d->nx + 2;
+    const long long n1 = (long long)d->nx * d->ny, n2 = d->nx;
+    SrcResid in;
+    in.f = f;
+    in.P = P;
+    in.fb = f_halo ? h1 + h2 + 1 : 0;
+    in.f1 = f_halo ? h1 : n1;
+    in.f2 = f_halo ? h2 : n2;
+    in.pb = p_halo ? h1 + h2 + 1 : 0;
+    in.p1 = p_halo ? h1 : n1;
+    in.p2 = p_halo ? h2 : n2;
+    in.a = d->a;
+    in.c = d->c;
+    in.d = d->dz;
+    in.idx2 = d->idx2;
+    in.idy2 = d->idy2;
+    in.ny = d->ny;
+    in.nz = d->nz;
+    return dispatch_plan(d->px, [&](auto fp) {
+        return launch_xf<decltype(fp), float2>(d, d->tw_x32, in, W32, s);
     });
 }
 

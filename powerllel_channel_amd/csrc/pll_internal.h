@@ -99,6 +99,15 @@ int pll_launch_xfwd32_copy(const pll_ppe_dev *d, const double *f_halo, double *F
                            hipStream_t s);
 int pll_launch_xbwd32_norm(const pll_ppe_dev *d, const void *W32, double *p_halo,
                            const double *eps, double *norm, hipStream_t s);
+/* x forward (FP32) of the residual r = f - L_h p formed in the load (f and p
+ * haloed when the flag is set, dense otherwise); same values as
+ * pll_launch_resid followed by the dense FP32 x forward */
+int pll_launch_xfwd32_resid(const pll_ppe_dev *d, const double *f, int f_halo, const double *P,
+                            int p_halo, void *W32, hipStream_t s);
+/* gauge[parts] := top-plane mean of p (haloed or dense), from the same
+ * per-block partial sums as pll_launch_resid computes */
+int pll_launch_gauge(const pll_ppe_dev *d, const double *P, int halo, double *gauge,
+                     hipStream_t s);
 /* residual with a dense f and a haloed p (see pll_launch_resid) */
 int pll_launch_resid_hp(const pll_ppe_dev *d, const double *F, const double *p_halo, float *r,
                         double *gauge, hipStream_t s);

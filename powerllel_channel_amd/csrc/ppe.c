@@ -662,15 +662,28 @@ int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream)
      * sweep measures the top-plane mean of P and the final write removes it. */
     double *gauge = h->d.pin_gauge ? (double *)h->gauge : NULL;
     const double *eps = gauge ? gauge + pll_gauge_parts(&h->d) : NULL;
+    const char *pf = getenv("PLL_REFINE_FUSED");
+    const int fused = !(pf && atoi(pf) == 0);
     for (int it = 1; it <= sweeps; it++) {
         const int last = it == sweeps;
-        if ((rc = launch_rc(pll_launch_resid(&h->d, d_var, (const double *)h->P64, (float *)h->r32,
-                                             last ? gauge : NULL, s),
-                            "residual")))
-            return rc;
-        if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_DENSE32, h->r32, h->W32, s),
-                            "x forward (fp32)")))
-            return rc;
+        if (fused) {
+            /* residual formed in the x transform's load */
+            if (last && gauge &&
+                (rc = launch_rc(pll_launch_gauge(&h->d, (const double *)h->P64, 0, gauge, s), "gauge")))
+                return rc;
+            if ((rc = launch_rc(pll_launch_xfwd32_resid(&h->d, d_var, 1, (const double *)h->P64, 0,
+                                                        h->W32, s),
+                                "x forward of the residual (fp32)")))
+                return rc;
+        } else {
+            if ((rc = launch_rc(pll_launch_resid(&h->d, d_var, (const double *)h->P64,
+                                                 (float *)h->r32, last ? gauge : NULL, s),
+                                "residual")))
+                return rc;
+            if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_DENSE32, h->r32, h->W32, s),
+                                "x forward (fp32)")))
+                return rc;
+        }
         if ((rc = spectral32(h, s))) return rc;
         if ((rc = launch_rc(pll_launch_xbwd32(&h->d, last ? PLL_DST_HALO_FROM_DENSE : PLL_DST_DENSE_ADD,
                                               h->W32, d_var, (double *)h->P64, last ? eps : NULL, s),
@@ -744,13 +757,23 @@ int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_swe
     double *gauge = h->d.pin_gauge ? (double *)h->gauge : NULL;
     const double *eps = gauge ? gauge + pll_gauge_parts(&h->d) : NULL;
     int it;
+    const char *pf = getenv("PLL_REFINE_FUSED");
+    const int fused = !(pf && atoi(pf) == 0);
     for (it = 1; it <= max_sweeps; it++) {
-        if ((rc = launch_rc(pll_launch_resid_hp(&h->d, F, d_var, (float *)h->r32, gauge, s),
-                            "residual")))
-            return rc;
-        if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_DENSE32, h->r32, h->W32, s),
-                            "x forward (fp32)")))
-            return rc;
+        if (fused) {
+            if (gauge && (rc = launch_rc(pll_launch_gauge(&h->d, d_var, 1, gauge, s), "gauge")))
+                return rc;
+            if ((rc = launch_rc(pll_launch_xfwd32_resid(&h->d, F, 0, d_var, 1, h->W32, s),
+                                "x forward of the residual (fp32)")))
+                return rc;
+        } else {
+            if ((rc = launch_rc(pll_launch_resid_hp(&h->d, F, d_var, (float *)h->r32, gauge, s),
+                                "residual")))
+                return rc;
+            if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_DENSE32, h->r32, h->W32, s),
+                                "x forward (fp32)")))
+                return rc;
+        }
         if ((rc = spectral32(h, s))) return rc;
         if ((rc = launch_rc(pll_launch_xbwd32_norm(&h->d, h->W32, d_var, eps, nrm, s),
                             "x backward (fp32)")))

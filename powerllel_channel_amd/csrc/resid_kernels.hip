@@ -101,6 +101,27 @@ __global__ void __launch_bounds__(TX *TY) k_resid(RArgs g)
     }
 }
 
+// The top-plane partial sums of k_resid alone (same blocks, same tree): for
+// the forms that fuse the residual into the x transform.
+__global__ void __launch_bounds__(TX *TY) k_top_parts(const double *__restrict__ P, Lay pl,
+                                                      int nx, int ny, int nz,
+                                                      double *__restrict__ gpart)
+{
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const double top =
+        (i < nx && j < ny) ? P[pl.base + (long long)(nz - 1) * pl.s1 + (long long)j * pl.s2 + i] : 0.0;
+    __shared__ double red[TX * TY];
+    const int t = threadIdx.y * TX + threadIdx.x;
+    red[t] = top;
+    __syncthreads();
+    for (int h = TX * TY / 2; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) gpart[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
 // eps = (sum of the n partials, in a fixed order) * inv
 __global__ void __launch_bounds__(256) k_gauge_final(const double *part, int n, double inv,
                                                      double *eps)
@@ -176,4 +197,18 @@ extern "C" int pll_launch_resid_hp(const pll_ppe_dev *d, const double *F, const 
                                    float *r, double *gauge, hipStream_t s)
 {
     return launch_resid(d, F, dense_lay(d), p_halo, halo_lay(d), r, gauge, s);
+}
+
+/* gauge[parts] := mean of p over the top plane, p haloed (halo != 0) or dense */
+extern "C" int pll_launch_gauge(const pll_ppe_dev *d, const double *P, int halo, double *gauge,
+                                hipStream_t s)
+{
+    const dim3 grid((d->nx + TX - 1) / TX, (d->ny + TY - 1) / TY);
+    hipLaunchKernelGGL(k_top_parts, grid, dim3(TX, TY), 0, s, P, halo ? halo_lay(d) : dense_lay(d),
+                       d->nx, d->ny, d->nz, gauge);
+    if (hipGetLastError() != hipSuccess) return -3;
+    const int n = pll_gauge_parts(d);
+    hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(256), 0, s, gauge, n,
+                       1.0 / ((double)d->nx * d->ny), gauge + n);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
