@@ -212,6 +212,19 @@ int pll_channel_step_body(pll_channel *c, pll_ppe *ppe, double *u, double *v,
  * Synchronises. */
 int pll_channel_wall_shear(pll_channel *c, const double *u, double *tau,
                            void *stream);
+/* calcAndWriteSkf, src/mod_monitor.f90:225-258, at the bottom (k = 1) and
+ * top (k = nz) wall: out = {u_tau, cf} bottom, then {u_tau, cf} top, with
+ * tau_w = re_inv (mean_ij u(i,j,kw) + u_crf) / (dzf(kw)/2), u_tau =
+ * sqrt(tau_w), cf = tau_w / (u_ref^2/2) (the records of monitor_skf_z1.out
+ * and monitor_skf_z2.out).  Synchronises. */
+int pll_channel_skin_friction(pll_channel *c, const double *u, double u_ref,
+                              double out[4], void *stream);
+/* extractAndWriteProbePoint, src/mod_monitor.f90:146-167: out = {u + u_crf,
+ * v, w, p} at the 1-based interior index (i,j,k) (probe_ijk); PLL_EINVAL for a
+ * point outside the domain.  Synchronises. */
+int pll_channel_probe(pll_channel *c, const double *u, const double *v,
+                      const double *w, const double *p, int i, int j, int k,
+                      double out[4], void *stream);
 /* Transposes of the step's linear(ised) pieces (each exact for the map the
  * forward kernel applies):
  *  bc_vel_adj / bc_p_adj: ghost adjoints added to the interior values the

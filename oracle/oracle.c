@@ -927,9 +927,10 @@ int orc_has_nan(long n, const double *q)
     return 0;
 }
 
-double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf)
-{ /* mod_monitor.f90:225-258 at both walls (k = 1 and k = nz), averaged */
-    double tau[2];
+void orc_wall_tau(const orc_grid_t *g, const double *u, double u_crf, double tau[2])
+{ /* calcAndWriteSkf, mod_monitor.f90:225-258, summed in the reference's order
+   * at the bottom (k = 1, dz = dzf(1)/2) and top (k = nz, dz = dzf(nz)/2)
+   * wall (the calls at mod_monitor.f90:168-171) */
     const int kw[2] = {1, g->nz};
     for (int s = 0; s < 2; s++) {
         const double dz = 0.5 * g->dzf[kw[s]];
@@ -938,6 +939,12 @@ double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf)
             for (int i = 1; i <= g->nx; i++) t = t + g->re_inv * (u[IX(g, i, j, kw[s])] + u_crf) / dz;
         tau[s] = t / g->nx / g->ny;
     }
+}
+
+double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf)
+{ /* the two walls' tau_w averaged */
+    double tau[2];
+    orc_wall_tau(g, u, u_crf, tau);
     return 0.5 * (tau[0] + tau[1]);
 }
 

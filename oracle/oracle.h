@@ -100,6 +100,7 @@ void orc_bc_vel(const orc_grid_t *g, double *u, double *v, double *w, double u_c
 void orc_bc_p(const orc_grid_t *g, double *p);
 double orc_max_div(const orc_grid_t *g, const double *u, const double *v,
                    const double *w, int ijk[3]);
+void orc_wall_tau(const orc_grid_t *g, const double *u, double u_crf, double tau[2]);
 double orc_wall_shear(const orc_grid_t *g, const double *u, double u_crf);
 double orc_max_cfl(const orc_grid_t *g, const double *u, const double *v,
                    const double *w, int ijk[3]);

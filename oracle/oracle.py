@@ -10,6 +10,7 @@ is the reference's Fortran array f(0:nx+1, 0:ny+1, 0:nz+1) (x fastest).
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -73,6 +74,7 @@ def lib():
         "orc_bc_p": (None, [gp, _dp]),
         "orc_max_div": (C.c_double, [gp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
         "orc_wall_shear": (C.c_double, [gp, _dp, C.c_double]),
+        "orc_wall_tau": (None, [gp, _dp, C.c_double, C.POINTER(C.c_double)]),
         "orc_max_cfl": (C.c_double, [gp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
         "orc_has_nan": (C.c_int, [C.c_long, _dp]),
         "orc_step": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
@@ -250,6 +252,22 @@ def has_nan(q):
 
 def wall_shear(G, u, u_crf=0.0):
     return lib().orc_wall_shear(G.ref(), u, u_crf)
+
+
+def skin_friction(G, u, u_ref=1.0, u_crf=0.0):
+    """calcAndWriteSkf (src/mod_monitor.f90:225-258): ((u_tau, cf) at k = 1,
+    (u_tau, cf) at k = nz); u_tau = sqrt(tau_w), cf = tau_w / (u_ref^2 / 2)."""
+    tau = (C.c_double * 2)()
+    lib().orc_wall_tau(G.ref(), u, u_crf, tau)
+    return tuple((math.sqrt(t) if t >= 0 else float("nan"), t / (0.5 * u_ref * u_ref))
+                 for t in tau)
+
+
+def probe(u, v, w, p, ijk, u_crf=0.0):
+    """extractAndWriteProbePoint (src/mod_monitor.f90:146-167) on haloed
+    (nz+2, ny+2, nx+2) arrays, 1-based interior ijk."""
+    i, j, k = ijk
+    return (u[k, j, i] + u_crf, v[k, j, i], w[k, j, i], p[k, j, i])
 
 
 def transform2crf(G, vel, vel_crf):

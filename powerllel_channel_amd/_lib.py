@@ -71,6 +71,8 @@ _SIGS = {
     "pll_channel_check_nan": (_ip, [_vp, _dp, C.POINTER(_ip), _vp]),
     "pll_channel_step_body": (_ip, [_vp, _vp] + [_dp] * 10 + [_vp]),
     "pll_channel_wall_shear": (_ip, [_vp, _dp, C.POINTER(_d), _vp]),
+    "pll_channel_skin_friction": (_ip, [_vp, _dp, _d, C.POINTER(_d), _vp]),
+    "pll_channel_probe": (_ip, [_vp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, C.POINTER(_d), _vp]),
     "pll_channel_bc_vel_adj": (_ip, [_vp, _dp, _dp, _dp, _vp]),
     "pll_channel_bc_p_adj": (_ip, [_vp, _dp, _vp]),
     "pll_channel_force_vel_adj": (_ip, [_vp, _dp, _dp, _dp, _vp]),

@@ -31,6 +31,8 @@ class Channel:
         self.lx, self.ly, self.lz = float(lx), float(ly), float(lz)
         self.dzf = np.ascontiguousarray(dzf_global, dtype=np.float64)
         self.nu, self.dt, self.u_crf = float(nu), float(dt), float(u_crf)
+        self.is_forced = tuple(bool(x) for x in is_forced)
+        self.vel_force = tuple(float(x) for x in vel_force)
         h = C.c_void_p()
         isf = (C.c_int * 3)(*[1 if x else 0 for x in is_forced])
         vf = (C.c_double * 3)(*[float(x) for x in vel_force])
@@ -133,6 +135,25 @@ class Channel:
         check(lib().pll_channel_step_body(self._h, ppe.handle, _p(u), _p(v), _p(w), _p(u1),
                                           _p(v1), _p(w1), _p(p), _p(bu), _p(bv), _p(bw),
                                           _stream_ptr(stream)))
+
+    def skin_friction(self, u, u_ref=1.0, stream=None):
+        """calcAndWriteSkf (src/mod_monitor.f90:225-258) at both walls:
+        ((u_tau, cf) at k = 1, (u_tau, cf) at k = nz)."""
+        self._chk(u)
+        out = (C.c_double * 4)()
+        check(lib().pll_channel_skin_friction(self._h, _p(u), float(u_ref), out,
+                                              _stream_ptr(stream)))
+        return (out[0], out[1]), (out[2], out[3])
+
+    def probe(self, u, v, w, p, ijk, stream=None):
+        """extractAndWriteProbePoint (src/mod_monitor.f90:146-167): (u + u_crf,
+        v, w, p) at the 1-based interior index ijk."""
+        self._chk(u, v, w, p)
+        out = (C.c_double * 4)()
+        i, j, k = (int(x) for x in ijk)
+        check(lib().pll_channel_probe(self._h, _p(u), _p(v), _p(w), _p(p), i, j, k, out,
+                                      _stream_ptr(stream)))
+        return tuple(out)
 
     def wall_shear(self, u, stream=None) -> float:
         """Wall shear of mod_monitor.f90:225-258, averaged over both walls."""

@@ -5,6 +5,7 @@
  * device), the physics constants and the small reduction buffers.  The
  * entry points mirror the reference's per-step routines (src/main.f90:131-212).
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -465,6 +466,53 @@ int pll_channel_wall_shear(pll_channel *c, const double *u, double *tau, void *s
     HIPCHK(hipMemcpyAsync(sums, c->scal + 6, sizeof sums, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     *tau = pll_wall_shear_from_sums(c, sums);
+    return PLL_OK;
+}
+
+/* calcAndWriteSkf, mod_monitor.f90:225-258, for each wall (k = 1 with
+ * dz = dzf(1)/2, k = nz with dz = dzf(nz)/2, mod_monitor.f90:168-171):
+ * tau_w = re_inv (mean_ij u(i,j,kw) + u_crf) / dz, u_tau = sqrt(tau_w),
+ * cf = tau_w / (u_ref^2 / 2).  The sums come from one on-device reduction. */
+int pll_channel_skin_friction(pll_channel *c, const double *u, double u_ref, double out[4],
+                              void *stream)
+{
+    CHK_C(c);
+    if (!u || !out) return pll_set_error(PLL_EINVAL, "NULL argument");
+    if (!(u_ref != 0.0)) return pll_set_error(PLL_EINVAL, "u_ref must be nonzero");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if ((rc = kl(pll_launch_wall_sum(&c->g, u, c->part, c->scal + 6, s), "skin friction")))
+        return rc;
+    double sums[2];
+    HIPCHK(hipMemcpyAsync(sums, c->scal + 6, sizeof sums, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double nxy = (double)c->g.nx * c->g.ny;
+    const int kw[2] = {1, c->g.nz};
+    for (int w = 0; w < 2; w++) {
+        const double tau = c->g.re_inv * (sums[w] / nxy + c->g.u_crf) / (0.5 * c->dzf_h[kw[w]]);
+        out[2 * w] = sqrt(tau); /* NaN for a negative tau_w, as Fortran's sqrt */
+        out[2 * w + 1] = tau / (0.5 * u_ref * u_ref);
+    }
+    return PLL_OK;
+}
+
+/* extractAndWriteProbePoint, mod_monitor.f90:146-167: u(i,j,k) + u_crf,
+ * v(i,j,k), w(i,j,k), p(i,j,k) at the 1-based interior index (i,j,k). */
+int pll_channel_probe(pll_channel *c, const double *u, const double *v, const double *w,
+                      const double *p, int i, int j, int k, double out[4], void *stream)
+{
+    CHK_C(c);
+    if (!u || !v || !w || !p || !out) return pll_set_error(PLL_EINVAL, "NULL argument");
+    if (i < 1 || i > c->g.nx || j < 1 || j > c->g.ny || k < 1 || k > c->g.nz)
+        return pll_set_error(PLL_EINVAL, "probe point (%d,%d,%d) outside 1..(%d,%d,%d)", i, j, k,
+                             c->g.nx, c->g.ny, c->g.nz);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t at = ((size_t)k * (c->g.ny + 2) + j) * (c->g.nx + 2) + i;
+    const double *f[4] = {u, v, w, p};
+    for (int q = 0; q < 4; q++)
+        HIPCHK(hipMemcpyAsync(out + q, f[q] + at, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out[0] += c->g.u_crf;
     return PLL_OK;
 }
 

@@ -258,6 +258,63 @@ def test_monitors_vs_oracle(orc, cuda):
         assert not passed and rep["nan"]
 
 
+def test_monitor_files_vs_oracle(orc, cuda, tmp_path):
+    """Skin friction per wall (calcAndWriteSkf), Mean_h of the forced
+    components and the probe point on the GPU against the oracle, then the
+    monitor files of two outputs against records formatted from the oracle's
+    values.  The wall sums reduce in another order than the reference's
+    running sum: 1e-13 relative; the probe is a copy: exact."""
+    import torch
+    from powerllel_channel_amd import Channel
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.monitor import Monitor, _record
+    from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields
+    nx, ny, nz, u_crf, u_ref = 64, 48, 40, 0.3, 1.25
+    print(f"[monitor files] {nx}x{ny}x{nz} u_crf={u_crf}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 3.0)
+    G = orc.OracleGrid(nx, ny, nz, LX, LY, LZ, g.dzf, 1 / 5600, 1e-3)
+    u, v, w = poiseuille_fields(g, 8)
+    u -= u_crf
+    orc.bc_vel(G, u, v, w, u_crf)
+    p = np.random.default_rng(9).standard_normal(g.shape)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, 1 / 5600, 1e-3, u_crf=u_crf,
+                 is_forced=(True, True, False), vel_force=(1.0, 0.0, 0.0))
+    du, dv, dw, dp = _dev(torch, cuda, u, v, w, p)
+    got = ch.skin_friction(du, u_ref)
+    ref = orc.skin_friction(G, u, u_ref=u_ref, u_crf=u_crf)
+    print(f"  skf gpu {got} oracle {ref}", flush=True)
+    for a, b in zip(got, ref):
+        for x, y in zip(a, b):
+            assert abs(x - y) <= 1e-13 * abs(y)
+    ijk = (17, 5, nz)
+    assert ch.probe(du, dv, dw, dp, ijk) == orc.probe(u, v, w, p, ijk, u_crf)
+    with pytest.raises(Exception):
+        ch.probe(du, dv, dw, dp, (nx + 1, 1, 1))
+    means = [orc.mean_h(G, u), orc.mean_h(G, v), 0.0]
+    assert abs(ch.mean_h(du) - means[0]) <= 1e-14 * abs(means[0])
+    with Monitor(ch, out_forcing=True, out_probe_point=True, probe_ijk=ijk,
+                 out_skf_z=(True, True), nt_out_moni=10, u_ref=u_ref,
+                 directory=str(tmp_path)) as mon:
+        for nt in (10, 15, 20):
+            mon.output(nt, du, dv, dw, dp)
+    want = {
+        "monitor_forcing.out": (means[0] + u_crf, means[1], means[2]),
+        "monitor_probe.out": orc.probe(u, v, w, p, ijk, u_crf),
+        "monitor_skf_z1.out": ref[0],
+        "monitor_skf_z2.out": ref[1],
+    }
+    for name, vals in want.items():
+        lines = open(tmp_path / name).read().splitlines()
+        recs = [ln for ln in lines if ln[:9].strip().isdigit()]
+        assert [r[:9] for r in recs] == ["       10", "       20"], name
+        for r in recs:
+            fields = [float(r[9 + 15 * q: 24 + 15 * q]) for q in range(len(vals))]
+            expect = [float(x) for x in _record(10, vals)[9:].split()]
+            for a, b in zip(fields, expect):  # 7 significant digits
+                assert abs(a - b) <= 1.01e-7 * abs(b), (name, r)
+    ch.close()
+
+
 def test_refined_steps_vs_oracle(orc, cuda):
     """RK2 steps with the PPE in the refined form, to a 1e-11 pressure
     tolerance, against the oracle's FP64 steps (the reference algorithm):
