@@ -60,6 +60,25 @@ def _run(cmd):
     return out
 
 
+HDF5 = os.environ.get("HDF5_ROOT", "/opt/conda")
+IO_LIB = os.path.join(LIBDIR, "libpllio.so")
+
+
+def build_io() -> str | None:
+    """libpllio.so: the HDF5 field files (host code, gcc).  Skipped, with the
+    reason returned as None, when no HDF5 development files are installed."""
+    if not os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = IO_LIB + ".tmp"
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall",
+          f"-I{os.path.join(HDF5, 'include')}", os.path.join(CSRC, "h5io.c"), "-o", tmp,
+          f"-L{os.path.join(HDF5, 'lib')}", "-lhdf5",
+          f"-Wl,-rpath,{os.path.join(HDF5, 'lib')}"])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -84,6 +103,8 @@ def build(verbose: bool = False) -> str:
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp,
           f"-L{ROCM}/lib", "-lamdhip64", "-lm"])
     os.replace(tmp, LIB)
+    if build_io() is None and verbose:
+        print(f"no HDF5 under {HDF5}: libpllio.so (field files) not built")
     return LIB
 
 
