@@ -222,7 +222,10 @@ static int choose_launch(pll_ppe *h)
  * pivots).  Returns 0 (streaming solve) if nothing fits. */
 static int part_fits(int nx, int nz, int m, int *M, int *P, int *S)
 {
-    static const int Ss[] = {16, 8, 4};
+    /* widest first: a tile row of 32 FP64 columns is 256 contiguous bytes,
+     * which HBM serves faster than 128-byte pieces (768x384x256: 0.270 ->
+     * 0.253 ms against 16 columns, same box) */
+    static const int Ss[] = {32, 16, 8, 4};
     if (m <= 0 || nz % m || !pll_trid_part_supported(m)) return 0;
     const int p = nz / m;
     if (p < 2 || p > 64 || (p & (p - 1))) return 0;

@@ -198,8 +198,8 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
 {
     extern __shared__ double tl[];
     const int lP = g.lP, P = 1 << lP, nz = g.nz;
-    constexpr int LS = S == 16 ? 4 : (S == 8 ? 3 : 2);
-    static_assert(S == 4 || S == 8 || S == 16, "columns per tile");
+    constexpr int LS = S == 32 ? 5 : (S == 16 ? 4 : (S == 8 ? 3 : 2));
+    static_assert(S == 4 || S == 8 || S == 16 || S == 32, "columns per tile");
     const int tid = threadIdx.x;
     const long long plane = (long long)g.nx * g.ny;
     const int nthr = S * P;
@@ -538,6 +538,7 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         case 4: PLL_TRID_LAUNCH(M_, 4, D_); break;                                              \
         case 8: PLL_TRID_LAUNCH(M_, 8, D_); break;                                              \
         case 16: PLL_TRID_LAUNCH(M_, 16, D_); break;                                            \
+        case 32: PLL_TRID_LAUNCH(M_, 32, D_); break;                                            \
         default: return -5;                                                                     \
     }
 #define PLL_TRID_CASE(M_)                                                                       \
