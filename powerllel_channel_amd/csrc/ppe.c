@@ -207,6 +207,10 @@ static int choose_launch(pll_ppe *h)
      * in L2), at most 80 KB of LDS so that two workgroups fit a CU */
     maxt = pll_fft_max_threads(py->E);
     int cols = 8;
+    {
+        const char *pc = getenv("PLL_Y_COLS"); /* tuning override of the y tile width */
+        if (pc && atoi(pc) > 0) cols = atoi(pc);
+    }
     if (cols > h->L) cols = h->L;
     while (cols > 1 && (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 81920)) cols--;
     if (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 81920)
