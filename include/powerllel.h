@@ -196,6 +196,15 @@ int pll_channel_check_nan(pll_channel *c, const double *q, int *has_nan,
 int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v,
                      double *w, double *u1, double *v1, double *w1, double *p,
                      void *stream);
+/* nsteps calls of pll_channel_step (src/main.f90:131-212 without the
+ * per-step checks and output), with the same results bit for bit.  With the
+ * FP64 solve and w not forced, each step's projection (correctVel, forcing,
+ * boundary fill) is formed inside the next step's RK1 as it reads the
+ * velocity, which saves one pass over the fields per step; that needs three
+ * more haloed fields of device memory (allocated on first use). */
+int pll_channel_run(pll_channel *c, pll_ppe *ppe, int nsteps, double *u,
+                    double *v, double *w, double *u1, double *v1, double *w1,
+                    double *p, void *stream);
 
 /* ------------------------------------------------------------------------
  * Adjoint of the projection step (no counterpart in the reference).

@@ -128,6 +128,13 @@ class Channel:
         check(lib().pll_channel_step(self._h, ppe.handle, _p(u), _p(v), _p(w), _p(u1), _p(v1),
                                      _p(w1), _p(p), _stream_ptr(stream)))
 
+    def run(self, ppe: PoissonSolver, nsteps, u, v, w, u1, v1, w1, p, stream=None):
+        """nsteps steps, equal to nsteps calls of step(); with the FP64 solve
+        the projection of each step is formed in the next step's RK1."""
+        self._chk(u, v, w, u1, v1, w1, p)
+        check(lib().pll_channel_run(self._h, ppe.handle, int(nsteps), _p(u), _p(v), _p(w),
+                                    _p(u1), _p(v1), _p(w1), _p(p), _stream_ptr(stream)))
+
     def step_body(self, ppe: PoissonSolver, u, v, w, u1, v1, w1, p, bu, bv, bw, stream=None):
         """step() with a body force (bu, bv, bw) added to the momentum
         right-hand side of both RK stages (not part of the reference)."""

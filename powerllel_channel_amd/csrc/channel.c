@@ -33,6 +33,7 @@ struct pll_channel {
     long long *sidx;   /* device: 1 index */
     double *dzf_h;     /* host copy of dzf_global (nz+2) */
     double lx, ly, lz;
+    double *run;       /* device: 3 haloed fields, the second velocity of pll_channel_run */
 };
 
 static int kl(int rc, const char *what)
@@ -50,6 +51,7 @@ int pll_channel_destroy(pll_channel *c)
     if (c->pidx) hipFree(c->pidx);
     if (c->scal) hipFree(c->scal);
     if (c->sidx) hipFree(c->sidx);
+    if (c->run) hipFree(c->run);
     free(c->dzf_h);
     free(c);
     return PLL_OK;
@@ -347,6 +349,93 @@ int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v, double 
                      double *v1, double *w1, double *p, void *stream)
 {
     return step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream);
+}
+
+/* m steps (m odd) whose projections are carried into the next step's RK1:
+ * step n leaves its RK2 output q (boundary-filled) and pressure p unprojected;
+ * RK1 of step n+1 reads q and p and projects every value it reads
+ * (pll_launch_rk1_corr), writing the projected interior to the other
+ * velocity set, where RK2 then runs in place.  The sets alternate between
+ * the caller's (u, v, w) = X and c->run = Y; RK2 of odd steps lands in X, so
+ * with m odd the final projection (correctVel + forcing + boundary fill) is
+ * made in place in X.  Same kernels and roundings as m calls of step_impl. */
+static int run_chain(pll_channel *c, pll_ppe *ppe, int m, double *u, double *v, double *w,
+                     double *u1, double *v1, double *w1, double *p, int fmask, hipStream_t s)
+{
+    const size_t n = (size_t)(c->g.nx + 2) * (c->g.ny + 2) * (c->g.nz + 2);
+    double *X[3] = {u, v, w}, *Y[3] = {c->run, c->run + n, c->run + 2 * n};
+    double **cur = X;
+    const int nb = pll_rk_blocks(&c->g);
+    const double *mean = fmask ? c->scal + 4 : NULL;
+    const double dti = 1.0 / c->g.dt;
+    int rc;
+    for (int step = 0; step < m; step++) {
+        if (step == 0) {
+            if ((rc = kl(pll_launch_rk1(&c->g, cur[0], cur[1], cur[2], u1, v1, w1, NULL, s), "rk1")))
+                return rc;
+        } else {
+            double **nxt = cur == X ? Y : X;
+            if ((rc = kl(pll_launch_rk1_corr(&c->g, cur[0], cur[1], cur[2], p, nxt[0], nxt[1],
+                                             nxt[2], u1, v1, w1, fmask, c->vel_force[0],
+                                             c->vel_force[1], mean, s),
+                         "rk1 (projecting)")))
+                return rc;
+            cur = nxt;
+        }
+        if ((rc = pll_channel_bc_vel(c, u1, v1, w1, s))) return rc;
+        if ((rc = kl(pll_launch_rk2(&c->g, cur[0], cur[1], cur[2], u1, v1, w1,
+                                    fmask ? c->part : NULL, fmask, NULL, s),
+                     "rk2")))
+            return rc;
+        for (int d = 0; d < 2; d++)
+            if ((fmask >> d) & 1)
+                if ((rc = kl(pll_launch_mean_final(nb, c->part + (size_t)d * nb, c->scal + 4 + d,
+                                                   c->g.nx, c->g.ny, s),
+                             "mean final")))
+                    return rc;
+        if ((rc = pll_channel_bc_vel(c, cur[0], cur[1], cur[2], s))) return rc;
+        pll_div_src q = {cur[0], cur[1], cur[2], c->g.dzf_inv, dti, dti * c->g.dx_inv,
+                         dti * c->g.dy_inv};
+        if ((rc = pll_ppe_solve_div(ppe, &q, p, s))) return rc;
+    }
+    if (cur != X) return pll_set_error(PLL_ESTATE, "internal: even step chain");
+    if ((rc = kl(pll_launch_correct_vel(&c->g, p, u, v, w, fmask, c->vel_force[0],
+                                        c->vel_force[1], mean, s),
+                 "correct_vel")))
+        return rc;
+    return pll_channel_bc_vel(c, u, v, w, s);
+}
+
+int pll_channel_run(pll_channel *c, pll_ppe *ppe, int nsteps, double *u, double *v, double *w,
+                    double *u1, double *v1, double *w1, double *p, void *stream)
+{
+    CHK_C(c);
+    if (!ppe) return pll_set_error(PLL_ESTATE, "PPE handle is NULL");
+    if (nsteps < 0) return pll_set_error(PLL_EINVAL, "nsteps = %d < 0", nsteps);
+    if (!u || !v || !w || !u1 || !v1 || !w1 || !p) return pll_set_error(PLL_EINVAL, "NULL field");
+    hipStream_t s = (hipStream_t)stream;
+    const int fmask = (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0);
+    const int chain = !c->is_forced[2] && pll_ppe_refinement(ppe) < 0;
+    int rc;
+    if (!chain || nsteps < 3) {
+        for (int i = 0; i < nsteps; i++)
+            if ((rc = step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream))) return rc;
+        return PLL_OK;
+    }
+    if (!c->run) {
+        const size_t n = (size_t)(c->g.nx + 2) * (c->g.ny + 2) * (c->g.nz + 2);
+        hipError_t e = hipMalloc((void **)&c->run, sizeof(double) * 3 * n);
+        if (e != hipSuccess) {
+            c->run = NULL;
+            return pll_set_error(PLL_ENOMEM, "hipMalloc of the run's second velocity failed: %s",
+                                 hipGetErrorString(e));
+        }
+    }
+    if (nsteps % 2 == 0) {  /* one plain step, then an odd chain */
+        if ((rc = step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream))) return rc;
+        nsteps--;
+    }
+    return run_chain(c, ppe, nsteps, u, v, w, u1, v1, w1, p, fmask, s);
 }
 
 int pll_channel_step_body_internal(pll_channel *c, pll_ppe *ppe, double *u, double *v,

@@ -142,6 +142,16 @@ typedef struct {
 int pll_launch_rk1(const pll_grid_dev *g, const double *u, const double *v,
                    const double *w, double *u1, double *v1, double *w1,
                    const double *const *body, hipStream_t s);
+/* RK1 from an unprojected velocity (u, v, w: the previous RK2 output after its
+ * boundary fill; p: its pressure with ghosts): every value read is projected
+ * on load as correctVel + forceVel (fmask bits with vfu, vfv and the device
+ * means mean[0..1]; mean NULL: no forcing) + updateBoundVel would leave it;
+ * the projected interior is written to (nu, nv, nw), the RK1 result to
+ * (u1, v1, w1). */
+int pll_launch_rk1_corr(const pll_grid_dev *g, const double *u, const double *v,
+                        const double *w, const double *p, double *nu, double *nv, double *nw,
+                        double *u1, double *v1, double *w1, int fmask, double vfu, double vfv,
+                        const double *mean, hipStream_t s);
 /* mpart non-NULL and fmask != 0: also the Mean_h partial sums of the new u
  * (fmask bit 0) and v (bit 1), pll_rk_blocks(g) of each, u's first. */
 int pll_launch_rk2(const pll_grid_dev *g, double *u, double *v, double *w,

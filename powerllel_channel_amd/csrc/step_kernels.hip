@@ -68,10 +68,27 @@ struct Body {
     const double *u, *v, *w;
 };
 
+__device__ __forceinline__ int wrap(int i, int n) { return i == 0 ? n : (i == n + 1 ? 1 : i); }
+
+// CORR (RK1 of a multi-step run): the velocity q is the previous step's RK2
+// output after its boundary fill, not yet projected.  Every value the
+// stencil reads is formed on load as correctVel (with the forcing shift) and
+// updateBoundVel would have left it, in the same roundings (k_correct,
+// k_bc_vel), from q and the pressure p with its ghosts; the projected
+// interior values are also written to (nu, nv, nw).
+struct Corr {
+    const double *p;
+    double *nu, *nv, *nw;
+    double dtdxi, dtdyi;
+    int fmask;
+    double vfu, vfv;
+    const double *mean;
+};
+
 constexpr int LTX = RK_TX + 2, LTY = RK_TY + 2, LPL = LTX * LTY;
 constexpr int NLD = (LPL + RK_TX * RK_TY - 1) / (RK_TX * RK_TY);
 
-template <bool SECOND, bool MEAN, bool BODY>
+template <bool SECOND, bool MEAN, bool BODY, bool CORR = false>
 __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
                                                      const double *__restrict__ qv,
                                                      const double *__restrict__ qw,
@@ -79,7 +96,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
                                                      double *__restrict__ ov,
                                                      double *__restrict__ ow,
                                                      double *__restrict__ mpart, int fmask,
-                                                     Body fb)
+                                                     Body fb, Corr cr = Corr{})
 {
     __shared__ double tl[2][3][LPL];
     int tx, ty, kc;
@@ -95,6 +112,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     // this thread's share of a plane tile: element q = tid + r * 256
     long long go[NLD];
     bool gk[NLD];
+    int gii[NLD], gjj[NLD];
 #pragma unroll
     for (int r = 0; r < NLD; ++r) {
         const int q = tid + r * RK_TX * RK_TY;
@@ -102,15 +120,91 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
         const int gi = i0 + col, gj = j0 + row;
         gk[r] = q < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
         go[r] = gi + sy * gj;
+        gii[r] = gk[r] ? wrap(gi, g.nx) : 1;
+        gjj[r] = gk[r] ? wrap(gj, g.ny) : 1;
     }
+    // CORR: projected value of field f at interior (i, j, k), as k_correct
+    const double shu = (CORR && (cr.fmask & 1)) ? cr.vfu - cr.mean[0] : 0.0;
+    const double shv = (CORR && (cr.fmask & 2)) ? cr.vfv - cr.mean[1] : 0.0;
+    auto proj = [&](int f, int ii, int jj, int kk) -> double {
+        const long long e0 = ii + sy * jj + sz * kk;
+        const double pc = cr.p[e0];
+        if (f == 0) {
+            double un = qu[e0] - (cr.p[e0 + 1] - pc) * cr.dtdxi;
+            if (cr.fmask & 1) un = un + shu;
+            return un;
+        }
+        if (f == 1) {
+            double vn = qv[e0] - (cr.p[e0 + sy] - pc) * cr.dtdyi;
+            if (cr.fmask & 2) vn = vn + shv;
+            return vn;
+        }
+        return qw[e0] - (cr.p[e0 + sz] - pc) * g.dt * g.dzc_inv[kk];
+    };
+    // ... and as k_bc_vel then leaves it at haloed (gi, gj, kk) (ii, jj wrapped)
+    auto filled = [&](int f, int ii, int jj, int kk) -> double {
+        if (kk == 0 || kk == g.nz + 1) {
+            if (f == 2) return 0.0;
+            const double bc = f == 0 ? 0.0 - g.u_crf : 0.0;
+            return 2.0 * bc - proj(f, ii, jj, kk == 0 ? 1 : g.nz);
+        }
+        if (f == 2 && kk == g.nz) return 0.0;
+        return proj(f, ii, jj, kk);
+    };
     double pre[3][NLD];
+    // CORR: the raw operands of the projection, loaded ahead like pre and
+    // combined only when the plane is written to LDS (so that the wait for
+    // them falls after the current plane's arithmetic)
+    double rp[NLD], rpx[NLD], rpy[NLD], rpz[NLD];
+    int kpre = 0;
     auto fetch = [&](int kp) {
+        kpre = kp;
+        if (CORR) {
+            const int ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
 #pragma unroll
-        for (int f = 0; f < 3; ++f)
+            for (int r = 0; r < NLD; ++r) {
+                const long long e0 = gii[r] + sy * gjj[r] + sz * ks;
+                pre[0][r] = gk[r] ? qu[e0] : 0.0;
+                pre[1][r] = gk[r] ? qv[e0] : 0.0;
+                pre[2][r] = gk[r] ? qw[e0] : 0.0;
+                rp[r] = gk[r] ? cr.p[e0] : 0.0;
+                rpx[r] = gk[r] ? cr.p[e0 + 1] : 0.0;
+                rpy[r] = gk[r] ? cr.p[e0 + sy] : 0.0;
+                rpz[r] = gk[r] ? cr.p[e0 + sz] : 0.0;
+            }
+        } else {
 #pragma unroll
-            for (int r = 0; r < NLD; ++r) pre[f][r] = gk[r] ? Q[f][go[r] + sz * kp] : 0.0;
+            for (int f = 0; f < 3; ++f)
+#pragma unroll
+                for (int r = 0; r < NLD; ++r) pre[f][r] = gk[r] ? Q[f][go[r] + sz * kp] : 0.0;
+        }
     };
     auto put = [&](int b) {
+        if (CORR) {
+            // projection (k_correct) of the plane's source values, then the
+            // fill (k_bc_vel) for the wall planes and w's wall row
+            const int kp = kpre, ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
+            const bool wall = kp == 0 || kp == g.nz + 1;
+            const double dzi = g.dzc_inv[ks], bcu = 0.0 - g.u_crf;
+#pragma unroll
+            for (int r = 0; r < NLD; ++r) {
+                double un = pre[0][r] - (rpx[r] - rp[r]) * cr.dtdxi;
+                if (cr.fmask & 1) un = un + shu;
+                double vn = pre[1][r] - (rpy[r] - rp[r]) * cr.dtdyi;
+                if (cr.fmask & 2) vn = vn + shv;
+                double wn = pre[2][r] - (rpz[r] - rp[r]) * g.dt * dzi;
+                if (wall) {
+                    un = 2.0 * bcu - un;
+                    vn = 2.0 * 0.0 - vn;
+                    wn = 0.0;
+                } else if (kp == g.nz) {
+                    wn = 0.0;
+                }
+                pre[0][r] = un;
+                pre[1][r] = vn;
+                pre[2][r] = wn;
+            }
+        }
 #pragma unroll
         for (int f = 0; f < 3; ++f)
 #pragma unroll
@@ -126,11 +220,19 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     long long e = i + sy * j + sz * k0;
     double u00m = 0.0, v00m = 0.0, w00m = 0.0, wp0m = 0.0, w0pm = 0.0;
     if (valid) {
-        u00m = qu[e - sz];
-        v00m = qv[e - sz];
-        w00m = qw[e - sz];
-        wp0m = qw[e - sz + 1];
-        w0pm = qw[e - sz + sy];
+        if (CORR) {
+            u00m = filled(0, i, j, k0 - 1);
+            v00m = filled(1, i, j, k0 - 1);
+            w00m = filled(2, i, j, k0 - 1);
+            wp0m = filled(2, wrap(i + 1, g.nx), j, k0 - 1);
+            w0pm = filled(2, i, wrap(j + 1, g.ny), k0 - 1);
+        } else {
+            u00m = qu[e - sz];
+            v00m = qv[e - sz];
+            w00m = qw[e - sz];
+            wp0m = qw[e - sz + 1];
+            w0pm = qw[e - sz + sy];
+        }
     }
     __syncthreads();
     double su = 0.0, sv = 0.0;
@@ -174,6 +276,11 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
                 ou[e] = n.u000 + g.dt * fu;
                 ov[e] = n.v000 + g.dt * fv;
                 ow[e] = n.w000 + g.dt * fw;
+                if (CORR) {
+                    cr.nu[e] = n.u000;
+                    cr.nv[e] = n.v000;
+                    cr.nw[e] = n.w000;
+                }
             }
             u00m = n.u000;
             v00m = n.v000;
@@ -304,8 +411,6 @@ __global__ void __launch_bounds__(256) k_force_shift(G g, double *__restrict__ q
     const double force = vel_force - mean[0];
     q[e] = q[e] + force;
 }
-
-__device__ __forceinline__ int wrap(int i, int n) { return i == 0 ? n : (i == n + 1 ? 1 : i); }
 
 // Ghost cells after updateHalo (y periodic, full x/z range), imposePeriodicBC
 // in x (full y/z range) and the wall conditions, in that order.  Every ghost
@@ -548,6 +653,30 @@ extern "C" int pll_launch_rk1(const pll_grid_dev *gd, const double *u, const dou
                            w, u1, v1, w1, nullptr, 0, fb);
         return ok();
     });
+}
+
+extern "C" int pll_launch_rk1_corr(const pll_grid_dev *gd, const double *u, const double *v,
+                                   const double *w, const double *p, double *nu, double *nv,
+                                   double *nw, double *u1, double *v1, double *w1, int fmask,
+                                   double vfu, double vfv, const double *mean, hipStream_t s)
+{
+    const G g = make_g(gd);
+    const Tiles t = rk_tiles(g);
+    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
+    Corr cr;
+    cr.p = p;
+    cr.nu = nu;
+    cr.nv = nv;
+    cr.nw = nw;
+    cr.dtdxi = g.dt * g.dx_inv;  // as pll_launch_correct_vel
+    cr.dtdyi = g.dt * g.dy_inv;
+    cr.fmask = mean ? fmask : 0;
+    cr.vfu = vfu;
+    cr.vfv = vfv;
+    cr.mean = mean;
+    hipLaunchKernelGGL((k_rk<false, false, false, true>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
+                       w1, nullptr, 0, Body{nullptr, nullptr, nullptr}, cr);
+    return ok();
 }
 
 extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, double *w,

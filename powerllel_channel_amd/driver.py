@@ -5,7 +5,8 @@ the mesh (initMesh, src/mod_mesh.f90:26-84,120-162), the PPE solver and the
 step, initialise the flow (initFlow, src/mod_initFlow.f90:19-82) or restart
 from checkpoint files (inputData), shift to the convecting frame
 (transform2CRF), fill the boundaries, then for nt = nt_start..nt_end: one
-time step (pll_channel_step, src/main.f90:137-208), the run-time check every
+time step (src/main.f90:137-208; the steps between two checks or outputs go
+to the device as one pll_channel_run call), the run-time check every
 nt_check steps (src/main.f90:213-235: NaN, CFL and divergence against their
 limits; a failed check prints the reference's messages and ends the run),
 the screen line every nt_out_scrn steps (format 998) and outputData (monitor
@@ -214,8 +215,14 @@ class Simulation:
         torch.cuda.synchronize()
         t0 = wtime = time.perf_counter()
         passed, nt = True, self.nt_start - 1
-        for nt in range(self.nt_start, nt_end + 1):
-            self.ch.step(self.ppe, self.u, self.v, self.w, self.u1, self.v1, self.w1, self.pr)
+        periods = (p.nt_check, p.nt_out_scrn, p.nt_out_moni, p.nt_out_inst, p.nt_out_save)
+        while nt < nt_end:
+            # the steps up to the next one with a check or an output run as one
+            # pll_channel_run call (the same results as step by step)
+            nxt = min([((nt + 1 + q - 1) // q) * q for q in periods] + [nt_end])
+            self.ch.run(self.ppe, nxt - nt, self.u, self.v, self.w, self.u1, self.v1, self.w1,
+                        self.pr)
+            nt = nxt
             if nt % p.nt_check == 0:
                 if not self.check(nt):
                     passed = False

@@ -317,6 +317,43 @@ def test_monitor_files_vs_oracle(orc, cuda, tmp_path):
     ch.close()
 
 
+@pytest.mark.parametrize("shape,forced,u_crf,nsteps", [
+    ((64, 48, 32), (True, False, False), 0.0, 5),
+    ((64, 48, 32), (True, True, False), 0.3, 4),
+    ((96, 54, 24), (False, False, False), 0.2, 3),
+    ((96, 54, 24), (True, False, False), 0.0, 6),
+])
+def test_run_equals_steps(orc, cuda, shape, forced, u_crf, nsteps):
+    """pll_channel_run (projection formed in the next step's RK1) against the
+    same number of pll_channel_step calls: bit for bit, every field, odd and
+    even step counts, with and without forcing, with a convecting frame."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields
+    nx, ny, nz = shape
+    print(f"[run vs steps] {shape} forced={forced} u_crf={u_crf} n={nsteps}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 3.0)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, 1 / 5600, 1e-3, u_crf=u_crf,
+                 is_forced=forced, vel_force=(1.0 - u_crf, 0.25, 0.0))
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    u, v, w = poiseuille_fields(g, 21)
+    u -= u_crf
+    z = np.zeros(g.shape)
+    a = _dev(torch, cuda, u, v, w, z, z, z, z)
+    ch.bc_vel(*a[:3])
+    b = [t.clone() for t in a]
+    for _ in range(nsteps):
+        ch.step(s, *a)
+    ch.run(s, nsteps, *b)
+    torch.cuda.synchronize()
+    for name, x, y in zip("u v w".split(), a[:3], b[:3]):
+        assert torch.equal(x, y), (name, float((x - y).abs().max()))
+    assert torch.equal(a[6][1:-1, 1:-1, 1:-1], b[6][1:-1, 1:-1, 1:-1])
+    s.close()
+    ch.close()
+
+
 @pytest.mark.parametrize("u_crf", [0.0, 0.3])
 def test_checkpoint_restart_bitwise(orc, cuda, tmp_path, u_crf):
     """Checkpoint after 2 of 4 steps (u + u_crf, v, w to HDF5 files in the
