@@ -351,54 +351,65 @@ int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v, double 
     return step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream);
 }
 
+/* A step of a chain up to its pressure solve: RK1 of the velocity `from`
+ * (from == NULL: the caller's projected X, plain RK1; otherwise an
+ * unprojected RK2 output, projected as RK1 reads it, pll_launch_rk1_corr,
+ * with the projected interior written to `to`), then RK2 in place in `to`,
+ * the forcing means, the boundary fill and the FP64 solve with calcRHS formed
+ * in its x transform.  The projection of `to` is left to the next step. */
+static int chain_step(pll_channel *c, pll_ppe *ppe, double *const *from, double *const *to,
+                      double *u1, double *v1, double *w1, double *p, int fmask, hipStream_t s)
+{
+    const int nb = pll_rk_blocks(&c->g);
+    const double *mean = fmask ? c->scal + 4 : NULL;
+    const double dti = 1.0 / c->g.dt;
+    int rc;
+    if (!from) {
+        if ((rc = kl(pll_launch_rk1(&c->g, to[0], to[1], to[2], u1, v1, w1, NULL, s), "rk1")))
+            return rc;
+    } else if ((rc = kl(pll_launch_rk1_corr(&c->g, from[0], from[1], from[2], p, to[0], to[1],
+                                            to[2], u1, v1, w1, fmask, c->vel_force[0],
+                                            c->vel_force[1], mean, s),
+                        "rk1 (projecting)"))) {
+        return rc;
+    }
+    if ((rc = pll_channel_bc_vel(c, u1, v1, w1, s))) return rc;
+    if ((rc = kl(pll_launch_rk2(&c->g, to[0], to[1], to[2], u1, v1, w1, fmask ? c->part : NULL,
+                                fmask, NULL, s),
+                 "rk2")))
+        return rc;
+    for (int d = 0; d < 2; d++)
+        if ((fmask >> d) & 1)
+            if ((rc = kl(pll_launch_mean_final(nb, c->part + (size_t)d * nb, c->scal + 4 + d,
+                                               c->g.nx, c->g.ny, s),
+                         "mean final")))
+                return rc;
+    if ((rc = pll_channel_bc_vel(c, to[0], to[1], to[2], s))) return rc;
+    pll_div_src q = {to[0], to[1], to[2], c->g.dzf_inv, dti, dti * c->g.dx_inv,
+                     dti * c->g.dy_inv};
+    return pll_ppe_solve_div(ppe, &q, p, s);
+}
+
 /* m steps (m odd) whose projections are carried into the next step's RK1:
- * step n leaves its RK2 output q (boundary-filled) and pressure p unprojected;
- * RK1 of step n+1 reads q and p and projects every value it reads
- * (pll_launch_rk1_corr), writing the projected interior to the other
- * velocity set, where RK2 then runs in place.  The sets alternate between
- * the caller's (u, v, w) = X and c->run = Y; RK2 of odd steps lands in X, so
- * with m odd the final projection (correctVel + forcing + boundary fill) is
- * made in place in X.  Same kernels and roundings as m calls of step_impl. */
+ * step n leaves its RK2 output (boundary-filled) and pressure p unprojected;
+ * RK1 of step n+1 reads both and projects every value it reads, writing the
+ * projected interior to the other velocity set, where RK2 then runs in place.
+ * The sets alternate between the caller's (u, v, w) = X and c->run = Y; the
+ * first step runs on X, so with m odd the last one ends on X and the final
+ * projection (correctVel + forcing + boundary fill) is made in place there.
+ * Same kernels and roundings as m calls of step_impl. */
 static int run_chain(pll_channel *c, pll_ppe *ppe, int m, double *u, double *v, double *w,
                      double *u1, double *v1, double *w1, double *p, int fmask, hipStream_t s)
 {
     const size_t n = (size_t)(c->g.nx + 2) * (c->g.ny + 2) * (c->g.nz + 2);
     double *X[3] = {u, v, w}, *Y[3] = {c->run, c->run + n, c->run + 2 * n};
-    double **cur = X;
-    const int nb = pll_rk_blocks(&c->g);
     const double *mean = fmask ? c->scal + 4 : NULL;
-    const double dti = 1.0 / c->g.dt;
     int rc;
-    for (int step = 0; step < m; step++) {
-        if (step == 0) {
-            if ((rc = kl(pll_launch_rk1(&c->g, cur[0], cur[1], cur[2], u1, v1, w1, NULL, s), "rk1")))
-                return rc;
-        } else {
-            double **nxt = cur == X ? Y : X;
-            if ((rc = kl(pll_launch_rk1_corr(&c->g, cur[0], cur[1], cur[2], p, nxt[0], nxt[1],
-                                             nxt[2], u1, v1, w1, fmask, c->vel_force[0],
-                                             c->vel_force[1], mean, s),
-                         "rk1 (projecting)")))
-                return rc;
-            cur = nxt;
-        }
-        if ((rc = pll_channel_bc_vel(c, u1, v1, w1, s))) return rc;
-        if ((rc = kl(pll_launch_rk2(&c->g, cur[0], cur[1], cur[2], u1, v1, w1,
-                                    fmask ? c->part : NULL, fmask, NULL, s),
-                     "rk2")))
-            return rc;
-        for (int d = 0; d < 2; d++)
-            if ((fmask >> d) & 1)
-                if ((rc = kl(pll_launch_mean_final(nb, c->part + (size_t)d * nb, c->scal + 4 + d,
-                                                   c->g.nx, c->g.ny, s),
-                             "mean final")))
-                    return rc;
-        if ((rc = pll_channel_bc_vel(c, cur[0], cur[1], cur[2], s))) return rc;
-        pll_div_src q = {cur[0], cur[1], cur[2], c->g.dzf_inv, dti, dti * c->g.dx_inv,
-                         dti * c->g.dy_inv};
-        if ((rc = pll_ppe_solve_div(ppe, &q, p, s))) return rc;
+    if ((rc = chain_step(c, ppe, NULL, X, u1, v1, w1, p, fmask, s))) return rc;
+    for (int i = 0; i < (m - 1) / 2; i++) {
+        if ((rc = chain_step(c, ppe, X, Y, u1, v1, w1, p, fmask, s))) return rc;
+        if ((rc = chain_step(c, ppe, Y, X, u1, v1, w1, p, fmask, s))) return rc;
     }
-    if (cur != X) return pll_set_error(PLL_ESTATE, "internal: even step chain");
     if ((rc = kl(pll_launch_correct_vel(&c->g, p, u, v, w, fmask, c->vel_force[0],
                                         c->vel_force[1], mean, s),
                  "correct_vel")))
@@ -415,7 +426,14 @@ int pll_channel_run(pll_channel *c, pll_ppe *ppe, int nsteps, double *u, double 
     if (!u || !v || !w || !u1 || !v1 || !w1 || !p) return pll_set_error(PLL_EINVAL, "NULL field");
     hipStream_t s = (hipStream_t)stream;
     const int fmask = (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0);
-    const int chain = !c->is_forced[2] && pll_ppe_refinement(ppe) < 0;
+    /* The projecting RK1 issues more loads per value than RK1 and correctVel
+     * together and needs enough workgroups to hide them: at 768x384x256
+     * (4608 tiles) the chain is 2% faster per step, at 192x192x128 (288
+     * tiles) 10% slower, so small grids take the per-step path. */
+    const char *pc = getenv("PLL_RUN_CHAIN"); /* tuning: 0 never, 1 always */
+    const int big = pll_rk_blocks(&c->g) >= 2048;
+    const int chain = !c->is_forced[2] && pll_ppe_refinement(ppe) < 0 &&
+                      (pc ? atoi(pc) != 0 : big);
     int rc;
     if (!chain || nsteps < 3) {
         for (int i = 0; i < nsteps; i++)

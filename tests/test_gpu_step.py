@@ -323,11 +323,13 @@ def test_monitor_files_vs_oracle(orc, cuda, tmp_path):
     ((96, 54, 24), (False, False, False), 0.2, 3),
     ((96, 54, 24), (True, False, False), 0.0, 6),
 ])
-def test_run_equals_steps(orc, cuda, shape, forced, u_crf, nsteps):
-    """pll_channel_run (projection formed in the next step's RK1) against the
-    same number of pll_channel_step calls: bit for bit, every field, odd and
-    even step counts, with and without forcing, with a convecting frame."""
+def test_run_equals_steps(orc, cuda, shape, forced, u_crf, nsteps, monkeypatch):
+    """pll_channel_run (projection formed in the next step's RK1; forced on
+    these small grids with PLL_RUN_CHAIN=1) against the same number of
+    pll_channel_step calls: bit for bit, every field, odd and even step
+    counts, with and without forcing, with a convecting frame."""
     import torch
+    monkeypatch.setenv("PLL_RUN_CHAIN", "1")
     from powerllel_channel_amd import Channel, PoissonSolver
     from powerllel_channel_amd.mesh import Grid
     from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields
@@ -350,6 +352,41 @@ def test_run_equals_steps(orc, cuda, shape, forced, u_crf, nsteps):
     for name, x, y in zip("u v w".split(), a[:3], b[:3]):
         assert torch.equal(x, y), (name, float((x - y).abs().max()))
     assert torch.equal(a[6][1:-1, 1:-1, 1:-1], b[6][1:-1, 1:-1, 1:-1])
+    s.close()
+    ch.close()
+
+
+@pytest.mark.parametrize("chain", ["1", "0"])
+def test_run_calls_vs_steps(orc, cuda, chain, monkeypatch):
+    """Four pll_channel_run calls in a row (5, 3, 7 and 5 steps), with
+    per-step calls and a Mean_h read-back between them, each checked bit for
+    bit against the same steps taken one call at a time; chain = "0": run()
+    as a loop of steps (PLL_RUN_CHAIN=0, the small-grid default)."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields
+    monkeypatch.setenv("PLL_RUN_CHAIN", chain)
+    nx, ny, nz = 64, 48, 32
+    print(f"[run calls] chain={chain}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 3.0)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, 1 / 5600, 1e-3)
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    u, v, w = poiseuille_fields(g, 22)
+    z = np.zeros(g.shape)
+    a = _dev(torch, cuda, u, v, w, z, z, z, z)
+    ch.bc_vel(*a[:3])
+    b = [t.clone() for t in a]
+    for n in (5, 3, 7, 5):
+        for _ in range(n):
+            ch.step(s, *a)
+        ch.run(s, n, *b)
+        torch.cuda.synchronize()
+        for name, x, y in zip("u v w".split(), a[:3], b[:3]):
+            assert torch.equal(x, y), (n, name, float((x - y).abs().max()))
+        ch.step(s, *a)  # eager work between the replays
+        ch.step(s, *b)
+        assert ch.mean_h(a[0]) == ch.mean_h(b[0])
     s.close()
     ch.close()
 
