@@ -87,6 +87,13 @@ struct Corr {
 
 constexpr int LTX = RK_TX + 2, LTY = RK_TY + 2, LPL = LTX * LTY;
 constexpr int NLD = (LPL + RK_TX * RK_TY - 1) / (RK_TX * RK_TY);
+// CORR: pressure tiles one cell wider on the high x and y sides (p(i+1),
+// p(j+1) of every velocity tile cell)
+constexpr int PTX = LTX + 1, PTY = LTY + 1, LPP = PTX * PTY;
+constexpr int NLP = (LPP + RK_TX * RK_TY - 1) / (RK_TX * RK_TY);
+
+// periodic index of a pressure tile cell: 0 -> n, n + 1 -> 1, n + 2 -> 2
+__device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n ? i - n : i); }
 
 template <bool SECOND, bool MEAN, bool BODY, bool CORR = false>
 __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
@@ -152,26 +159,42 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
         return proj(f, ii, jj, kk);
     };
     double pre[3][NLD];
-    // CORR: the raw operands of the projection, loaded ahead like pre and
-    // combined only when the plane is written to LDS (so that the wait for
-    // them falls after the current plane's arithmetic)
-    double rp[NLD], rpx[NLD], rpy[NLD], rpz[NLD];
-    int kpre = 0;
+    // CORR: the plane's raw velocity (at the wrapped source cells) is loaded
+    // ahead like pre; the pressure goes through two LDS tiles, p of the source
+    // plane and of the plane above (w's projection), the latter loaded ahead
+    // into pnext; the projection is formed when the plane is written to LDS
+    __shared__ double tp[CORR ? 2 : 1][CORR ? LPP : 1];
+    long long gpo[NLP];
+    bool gpk[NLP];
+    double pnext[NLP];
+    int kpre = 0, pcur = 0;
+    if (CORR) {
+#pragma unroll
+        for (int r = 0; r < NLP; ++r) {
+            const int q = tid + r * RK_TX * RK_TY;
+            const int row = q / PTX, col = q - row * PTX;
+            const int gi = i0 + col, gj = j0 + row;
+            gpk[r] = q < LPP && gi <= g.nx + 2 && gj <= g.ny + 2;
+            gpo[r] = gpk[r] ? wrap_p(gi, g.nx) + sy * wrap_p(gj, g.ny) : 0;
+            // p of the chunk's first plane, for its first projection
+            if (q < LPP) tp[0][q] = gpk[r] ? cr.p[gpo[r] + sz * k0] : 0.0;
+        }
+    }
     auto fetch = [&](int kp) {
         kpre = kp;
         if (CORR) {
             const int ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
+            const bool above = kp >= 1 && kp <= g.nz - 1;
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
                 const long long e0 = gii[r] + sy * gjj[r] + sz * ks;
                 pre[0][r] = gk[r] ? qu[e0] : 0.0;
                 pre[1][r] = gk[r] ? qv[e0] : 0.0;
                 pre[2][r] = gk[r] ? qw[e0] : 0.0;
-                rp[r] = gk[r] ? cr.p[e0] : 0.0;
-                rpx[r] = gk[r] ? cr.p[e0 + 1] : 0.0;
-                rpy[r] = gk[r] ? cr.p[e0 + sy] : 0.0;
-                rpz[r] = gk[r] ? cr.p[e0 + sz] : 0.0;
             }
+#pragma unroll
+            for (int r = 0; r < NLP; ++r)
+                pnext[r] = (above && gpk[r]) ? cr.p[gpo[r] + sz * (kp + 1)] : 0.0;
         } else {
 #pragma unroll
             for (int f = 0; f < 3; ++f)
@@ -182,17 +205,33 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     auto put = [&](int b) {
         if (CORR) {
             // projection (k_correct) of the plane's source values, then the
-            // fill (k_bc_vel) for the wall planes and w's wall row
+            // fill (k_bc_vel) for the wall planes and w's wall row.  tp[pcur]
+            // holds p of the source plane ks; p of the plane above goes to the
+            // other tile and becomes the source plane's p for the next put
             const int kp = kpre, ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
             const bool wall = kp == 0 || kp == g.nz + 1;
+            const bool above = kp >= 1 && kp <= g.nz - 1;
             const double dzi = g.dzc_inv[ks], bcu = 0.0 - g.u_crf;
+            if (above) {
+#pragma unroll
+                for (int r = 0; r < NLP; ++r) {
+                    const int q = tid + r * RK_TX * RK_TY;
+                    if (q < LPP) tp[pcur ^ 1][q] = pnext[r];
+                }
+            }
+            __syncthreads();
+            const double *Pc = tp[pcur], *Pn = tp[pcur ^ 1];
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
-                double un = pre[0][r] - (rpx[r] - rp[r]) * cr.dtdxi;
+                const int q = tid + r * RK_TX * RK_TY;
+                const int row = q / LTX, col = q - row * LTX;
+                const int o = q < LPL ? row * PTX + col : 0;
+                const double pc = Pc[o];
+                double un = pre[0][r] - (Pc[o + 1] - pc) * cr.dtdxi;
                 if (cr.fmask & 1) un = un + shu;
-                double vn = pre[1][r] - (rpy[r] - rp[r]) * cr.dtdyi;
+                double vn = pre[1][r] - (Pc[o + PTX] - pc) * cr.dtdyi;
                 if (cr.fmask & 2) vn = vn + shv;
-                double wn = pre[2][r] - (rpz[r] - rp[r]) * g.dt * dzi;
+                double wn = above ? pre[2][r] - (Pn[o] - pc) * g.dt * dzi : 0.0;
                 if (wall) {
                     un = 2.0 * bcu - un;
                     vn = 2.0 * 0.0 - vn;
@@ -204,6 +243,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
                 pre[1][r] = vn;
                 pre[2][r] = wn;
             }
+            if (above) pcur ^= 1;
         }
 #pragma unroll
         for (int f = 0; f < 3; ++f)
@@ -215,6 +255,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     };
     fetch(k0);
     put(0);
+    if (CORR) __syncthreads();  // put(0)'s pressure reads are done before put(1) writes a tile
     fetch(k0 + 1);
     put(1);
     long long e = i + sy * j + sz * k0;
