@@ -85,18 +85,20 @@ struct Corr {
     const double *mean;
 };
 
-constexpr int LTX = RK_TX + 2, LTY = RK_TY + 2, LPL = LTX * LTY;
-constexpr int NLD = (LPL + RK_TX * RK_TY - 1) / (RK_TX * RK_TY);
+// The projecting RK1 (CORR) uses tiles of 64 x 8 columns: its pressure tiles
+// add to the halo, so taller tiles pay (768x384x256: 4.60 -> 4.47 ms per step),
+// while the other RK kernels are faster with 64 x 4 (4.74 vs 4.82 ms).
+constexpr int RK_TY_CORR = 8;
+constexpr int LTX = RK_TX + 2;
 // CORR: pressure tiles one cell wider on the high x and y sides (p(i+1),
 // p(j+1) of every velocity tile cell)
-constexpr int PTX = LTX + 1, PTY = LTY + 1, LPP = PTX * PTY;
-constexpr int NLP = (LPP + RK_TX * RK_TY - 1) / (RK_TX * RK_TY);
+constexpr int PTX = LTX + 1;
 
 // periodic index of a pressure tile cell: 0 -> n, n + 1 -> 1, n + 2 -> 2
 __device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n ? i - n : i); }
 
-template <bool SECOND, bool MEAN, bool BODY, bool CORR = false>
-__global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
+template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY>
+__global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
                                                      const double *__restrict__ qv,
                                                      const double *__restrict__ qw,
                                                      double *__restrict__ ou,
@@ -105,12 +107,15 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
                                                      double *__restrict__ mpart, int fmask,
                                                      Body fb, Corr cr = Corr{})
 {
+    constexpr int LTY = TY + 2, LPL = LTX * LTY, NT = RK_TX * TY;
+    constexpr int NLD = (LPL + NT - 1) / NT;
+    constexpr int PTY = LTY + 1, LPP = PTX * PTY, NLP = (LPP + NT - 1) / NT;
     __shared__ double tl[2][3][LPL];
     int tx, ty, kc;
     rk_tile(t, tx, ty, kc);
     const int tid = threadIdx.x;
     const int lx = tid % RK_TX, ly = tid / RK_TX;
-    const int i0 = tx * RK_TX, j0 = ty * RK_TY;  // haloed index of the tile's halo corner
+    const int i0 = tx * RK_TX, j0 = ty * TY;  // haloed index of the tile's halo corner
     const int i = i0 + lx + 1, j = j0 + ly + 1;
     const bool valid = i <= g.nx && j <= g.ny;
     const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
@@ -122,7 +127,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     int gii[NLD], gjj[NLD];
 #pragma unroll
     for (int r = 0; r < NLD; ++r) {
-        const int q = tid + r * RK_TX * RK_TY;
+        const int q = tid + r * NT;
         const int row = q / LTX, col = q - row * LTX;
         const int gi = i0 + col, gj = j0 + row;
         gk[r] = q < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
@@ -171,7 +176,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
     if (CORR) {
 #pragma unroll
         for (int r = 0; r < NLP; ++r) {
-            const int q = tid + r * RK_TX * RK_TY;
+            const int q = tid + r * NT;
             const int row = q / PTX, col = q - row * PTX;
             const int gi = i0 + col, gj = j0 + row;
             gpk[r] = q < LPP && gi <= g.nx + 2 && gj <= g.ny + 2;
@@ -215,7 +220,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
             if (above) {
 #pragma unroll
                 for (int r = 0; r < NLP; ++r) {
-                    const int q = tid + r * RK_TX * RK_TY;
+                    const int q = tid + r * NT;
                     if (q < LPP) tp[pcur ^ 1][q] = pnext[r];
                 }
             }
@@ -223,7 +228,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
             const double *Pc = tp[pcur], *Pn = tp[pcur ^ 1];
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
-                const int q = tid + r * RK_TX * RK_TY;
+                const int q = tid + r * NT;
                 const int row = q / LTX, col = q - row * LTX;
                 const int o = q < LPL ? row * PTX + col : 0;
                 const double pc = Pc[o];
@@ -249,7 +254,7 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
         for (int f = 0; f < 3; ++f)
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
-                const int q = tid + r * RK_TX * RK_TY;
+                const int q = tid + r * NT;
                 if (q < LPL) tl[b][f][q] = pre[f][r];
             }
     };
@@ -334,11 +339,11 @@ __global__ void __launch_bounds__(RK_TX *RK_TY) k_rk(G g, Tiles t, const double 
         __syncthreads();
     }
     if (MEAN) {
-        __shared__ double red[2][RK_TX * RK_TY];
+        __shared__ double red[2][NT];
         red[0][tid] = su;
         red[1][tid] = sv;
         __syncthreads();
-        for (int h = RK_TX * RK_TY / 2; h > 0; h >>= 1) {
+        for (int h = NT / 2; h > 0; h >>= 1) {
             if (tid < h) {
                 red[0][tid] += red[0][tid + h];
                 red[1][tid] += red[1][tid + h];
@@ -649,12 +654,12 @@ __global__ void __launch_bounds__(256) k_nan_part(const double *__restrict__ q, 
 
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -3; }
 
-Tiles rk_tiles(const G &g)
+Tiles rk_tiles(const G &g, int ty = RK_TY)
 {
     Tiles t;
     t.kc = 64;
     t.ntx = (g.nx + RK_TX - 1) / RK_TX;
-    t.nty = (g.ny + RK_TY - 1) / RK_TY;
+    t.nty = (g.ny + ty - 1) / ty;
     t.nkc = (g.nz + t.kc - 1) / t.kc;
     t.xcd = ((long long)t.ntx * t.nty * t.nkc) % 8 == 0;
     return t;
@@ -702,8 +707,8 @@ extern "C" int pll_launch_rk1_corr(const pll_grid_dev *gd, const double *u, cons
                                    double vfu, double vfv, const double *mean, hipStream_t s)
 {
     const G g = make_g(gd);
-    const Tiles t = rk_tiles(g);
-    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
+    const Tiles t = rk_tiles(g, RK_TY_CORR);
+    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY_CORR);
     Corr cr;
     cr.p = p;
     cr.nu = nu;
@@ -715,7 +720,7 @@ extern "C" int pll_launch_rk1_corr(const pll_grid_dev *gd, const double *u, cons
     cr.vfu = vfu;
     cr.vfv = vfv;
     cr.mean = mean;
-    hipLaunchKernelGGL((k_rk<false, false, false, true>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
+    hipLaunchKernelGGL((k_rk<false, false, false, true, RK_TY_CORR>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
                        w1, nullptr, 0, Body{nullptr, nullptr, nullptr}, cr);
     return ok();
 }
