@@ -355,8 +355,9 @@ int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v, double 
  * (from == NULL: the caller's projected X, plain RK1; otherwise an
  * unprojected RK2 output, projected as RK1 reads it, pll_launch_rk1_corr,
  * with the projected interior written to `to`), then RK2 in place in `to`,
- * the forcing means, the boundary fill and the FP64 solve with calcRHS formed
- * in its x transform.  The projection of `to` is left to the next step. */
+ * the forcing means and the FP64 solve with calcRHS formed in its x
+ * transform.  The projection of `to` and its boundary fill are left to the
+ * next step (or to the chain's final projection). */
 static int chain_step(pll_channel *c, pll_ppe *ppe, double *const *from, double *const *to,
                       double *u1, double *v1, double *w1, double *p, int fmask, hipStream_t s)
 {
@@ -384,9 +385,10 @@ static int chain_step(pll_channel *c, pll_ppe *ppe, double *const *from, double 
                                                c->g.nx, c->g.ny, s),
                          "mean final")))
                 return rc;
-    if ((rc = pll_channel_bc_vel(c, to[0], to[1], to[2], s))) return rc;
+    /* no boundary fill of `to`: the solve's calcRHS and the next step's RK1
+     * read the values the fill would leave from the interior */
     pll_div_src q = {to[0], to[1], to[2], c->g.dzf_inv, dti, dti * c->g.dx_inv,
-                     dti * c->g.dy_inv};
+                     dti * c->g.dy_inv, 1};
     return pll_ppe_solve_div(ppe, &q, p, s);
 }
 

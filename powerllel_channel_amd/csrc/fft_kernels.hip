@@ -495,6 +495,33 @@ struct SrcDiv {  // PPE right-hand side formed from the velocity on the fly
         return mk<V>((typename Sc<V>::T)r0, (typename Sc<V>::T)r1);
     }
 };
+struct SrcDivUnfilled {  // SrcDiv on a velocity whose boundary fill is not made
+    pll_div_src q;
+    long long off1, off2;
+    int nx, ny, nz;
+    template <typename V>
+    __device__ __forceinline__ V load(long long, long long j, long long k, int m, int) const
+    {
+        // the values updateBoundVel would have left: x and y periodic copies,
+        // w = 0 on the wall rows (k = 0 below the first cell, k = nz on top)
+#pragma clang fp contract(off)
+        const long long e = (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m;
+        const d2u uc = *reinterpret_cast<const d2u *>(q.u + e);
+        const double um = q.u[m == 0 ? e - 1 + nx : e - 1];
+        const d2u vc = *reinterpret_cast<const d2u *>(q.v + e);
+        const d2u vs = *reinterpret_cast<const d2u *>(q.v + (j == 0 ? e + (long long)(ny - 1) * off2
+                                                                     : e - off2));
+        d2u wc = *reinterpret_cast<const d2u *>(q.w + e);
+        if (k == nz - 1) wc.x = wc.y = 0.0;
+        d2u wb;
+        wb.x = wb.y = 0.0;
+        if (k > 0) wb = *reinterpret_cast<const d2u *>(q.w + e - off1);
+        const double dzi = q.dzf_inv[k + 1];
+        const double r0 = (uc.x - um) * q.dtidxi + (vc.x - vs.x) * q.dtidyi + (wc.x - wb.x) * q.dti * dzi;
+        const double r1 = (uc.y - uc.x) * q.dtidxi + (vc.y - vs.y) * q.dtidyi + (wc.y - wb.y) * q.dti * dzi;
+        return mk<V>((typename Sc<V>::T)r0, (typename Sc<V>::T)r1);
+    }
+};
 
 // Destinations of the x backward transform; (a, b) = normalised (x_2m, x_2m+1).
 struct DstHalo64 {  // haloed FP64 array interior := value
@@ -1067,6 +1094,13 @@ extern "C" int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, h
 extern "C" int pll_launch_xfwd_div(const pll_ppe_dev *d, const pll_div_src *q, void *W,
                                    hipStream_t s)
 {
+    if (q->unfilled) {
+        const SrcDivUnfilled src{*q, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2, d->nx, d->ny,
+                                 d->nz};
+        return dispatch_plan(d->px, [&](auto f) {
+            return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s);
+        });
+    }
     const SrcDiv src{*q, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s);

@@ -67,6 +67,10 @@ int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t 
 typedef struct {
     const double *u, *v, *w, *dzf_inv;
     double dti, dtidxi, dtidyi;
+    /* 1: the velocity's boundary fill has not been made; the transform reads
+     * what it would have left (u(0) = u(nx), v(j=0) = v(ny), w = 0 on the
+     * wall rows k = 0 and nz) */
+    int unfilled;
 } pll_div_src;
 int pll_launch_xfwd_div(const pll_ppe_dev *d, const pll_div_src *q, void *W, hipStream_t s);
 
