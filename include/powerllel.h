@@ -192,7 +192,10 @@ int pll_channel_check_nan(pll_channel *c, const double *q, int *has_nan,
  * (same arithmetic).  Forcing of u and/or v takes Mean_h from the RK2 output
  * and shifts inside correctVel: the x (y) pressure-gradient correction sums
  * to zero over a periodic row, so this is the same mean up to summation
- * roundoff.  A forced w, or the refined solve, uses the separate passes. */
+ * roundoff.  A forced w, or the refined solve, uses the separate passes.
+ * (u1, v1, w1) is work space for the RK1 stage: RK2 reads its boundary
+ * values from its interior, so the step leaves its boundary fill (ghost
+ * cells, w1's wall row) unmade. */
 int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v,
                      double *w, double *u1, double *v1, double *w1, double *p,
                      void *stream);

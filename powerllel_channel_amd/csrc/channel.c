@@ -148,7 +148,7 @@ int pll_channel_rk2(pll_channel *c, double *u, double *v, double *w, const doubl
                     const double *v1, const double *w1, void *stream)
 {
     CHK_C(c);
-    return kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, NULL, (hipStream_t)stream),
+    return kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, NULL, 0, NULL, 0, (hipStream_t)stream),
               "rk2");
 }
 
@@ -309,9 +309,9 @@ static int step_impl(pll_channel *c, pll_ppe *ppe, double *u, double *v, double 
     const int none = !fmask && !c->is_forced[2];
     const int nb = pll_rk_blocks(&c->g);
     if ((rc = kl(pll_launch_rk1(&c->g, u, v, w, u1, v1, w1, body, s), "rk1"))) return rc;
-    if ((rc = pll_channel_bc_vel(c, u1, v1, w1, stream))) return rc;
+    /* RK2 reads u1's boundary values from its interior (no fill pass) */
     if ((rc = kl(pll_launch_rk2(&c->g, u, v, w, u1, v1, w1, fuse ? c->part : NULL,
-                                fuse ? fmask : 0, body, s),
+                                fuse ? fmask : 0, body, 1, s),
                  "rk2")))
         return rc;
     if (fuse)
@@ -374,9 +374,8 @@ static int chain_step(pll_channel *c, pll_ppe *ppe, double *const *from, double 
                         "rk1 (projecting)"))) {
         return rc;
     }
-    if ((rc = pll_channel_bc_vel(c, u1, v1, w1, s))) return rc;
     if ((rc = kl(pll_launch_rk2(&c->g, to[0], to[1], to[2], u1, v1, w1, fmask ? c->part : NULL,
-                                fmask, NULL, s),
+                                fmask, NULL, 1, s),
                  "rk2")))
         return rc;
     for (int d = 0; d < 2; d++)

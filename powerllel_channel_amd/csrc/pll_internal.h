@@ -158,9 +158,11 @@ int pll_launch_rk1_corr(const pll_grid_dev *g, const double *u, const double *v,
                         const double *mean, hipStream_t s);
 /* mpart non-NULL and fmask != 0: also the Mean_h partial sums of the new u
  * (fmask bit 0) and v (bit 1), pll_rk_blocks(g) of each, u's first. */
+/* fill != 0: (u1, v1, w1) have no boundary fill; RK2 reads the values the
+ * fill (updateBoundVel) would leave from their interior. */
 int pll_launch_rk2(const pll_grid_dev *g, double *u, double *v, double *w,
                    const double *u1, const double *v1, const double *w1, double *mpart,
-                   int fmask, const double *const *body, hipStream_t s);
+                   int fmask, const double *const *body, int fill, hipStream_t s);
 int pll_rk_blocks(const pll_grid_dev *g);
 int pll_launch_calc_rhs(const pll_grid_dev *g, const double *u, const double *v,
                         const double *w, double *p, hipStream_t s);

@@ -97,7 +97,9 @@ constexpr int PTX = LTX + 1;
 // periodic index of a pressure tile cell: 0 -> n, n + 1 -> 1, n + 2 -> 2
 __device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n ? i - n : i); }
 
-template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY>
+// FILL: q's boundary fill has not been made; the values it would leave
+// (k_bc_vel) are read from the interior instead of the ghost cells.
+template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY, bool FILL = false>
 __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
                                                      const double *__restrict__ qv,
                                                      const double *__restrict__ qw,
@@ -200,6 +202,13 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
 #pragma unroll
             for (int r = 0; r < NLP; ++r)
                 pnext[r] = (above && gpk[r]) ? cr.p[gpo[r] + sz * (kp + 1)] : 0.0;
+        } else if (FILL) {
+            const int ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+#pragma unroll
+                for (int r = 0; r < NLD; ++r)
+                    pre[f][r] = gk[r] ? Q[f][gii[r] + sy * gjj[r] + sz * ks] : 0.0;
         } else {
 #pragma unroll
             for (int f = 0; f < 3; ++f)
@@ -249,6 +258,20 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
                 pre[2][r] = wn;
             }
             if (above) pcur ^= 1;
+        } else if (FILL) {
+            const int kp = kpre;
+            if (kp == 0 || kp == g.nz + 1) {
+                const double bcu = 0.0 - g.u_crf;
+#pragma unroll
+                for (int r = 0; r < NLD; ++r) {
+                    pre[0][r] = 2.0 * bcu - pre[0][r];
+                    pre[1][r] = 2.0 * 0.0 - pre[1][r];
+                    pre[2][r] = 0.0;
+                }
+            } else if (kp == g.nz) {
+#pragma unroll
+                for (int r = 0; r < NLD; ++r) pre[2][r] = 0.0;
+            }
         }
 #pragma unroll
         for (int f = 0; f < 3; ++f)
@@ -266,7 +289,23 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
     long long e = i + sy * j + sz * k0;
     double u00m = 0.0, v00m = 0.0, w00m = 0.0, wp0m = 0.0, w0pm = 0.0;
     if (valid) {
-        if (CORR) {
+        if (FILL) {
+            // plane k0 - 1 as the fill leaves it: a wall plane for k0 = 1
+            auto fq = [&](const double *q, int f, int ii, int jj) -> double {
+                const int kk = k0 - 1;
+                if (kk == 0) {
+                    if (f == 2) return 0.0;
+                    const double bc = f == 0 ? 0.0 - g.u_crf : 0.0;
+                    return 2.0 * bc - q[ii + sy * jj + sz * 1];
+                }
+                return q[ii + sy * jj + sz * kk];  // kk < nz: no wall row here
+            };
+            u00m = fq(qu, 0, i, j);
+            v00m = fq(qv, 1, i, j);
+            w00m = fq(qw, 2, i, j);
+            wp0m = fq(qw, 2, wrap(i + 1, g.nx), j);
+            w0pm = fq(qw, 2, i, wrap(j + 1, g.ny));
+        } else if (CORR) {
             u00m = filled(0, i, j, k0 - 1);
             v00m = filled(1, i, j, k0 - 1);
             w00m = filled(2, i, j, k0 - 1);
@@ -727,7 +766,7 @@ extern "C" int pll_launch_rk1_corr(const pll_grid_dev *gd, const double *u, cons
 
 extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, double *w,
                               const double *u1, const double *v1, const double *w1,
-                              double *mpart, int fmask, const double *const *body,
+                              double *mpart, int fmask, const double *const *body, int fill,
                               hipStream_t s)
 {
     const G g = make_g(gd);
@@ -735,10 +774,11 @@ extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, doub
     const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
     const Body fb = body ? Body{body[0], body[1], body[2]} : Body{nullptr, nullptr, nullptr};
     const bool mean = mpart && fmask;
-    return with_flags(mean, body != nullptr, false, [&](auto M, auto B, auto) {
-        hipLaunchKernelGGL((k_rk<true, decltype(M)::value, decltype(B)::value>), grid, blk, 0, s,
-                           g, t, u1, v1, w1, u, v, w, mean ? mpart : nullptr, mean ? fmask : 0,
-                           fb);
+    return with_flags(mean, body != nullptr, fill != 0, [&](auto M, auto B, auto F) {
+        hipLaunchKernelGGL((k_rk<true, decltype(M)::value, decltype(B)::value, false, RK_TY,
+                                 decltype(F)::value>),
+                           grid, blk, 0, s, g, t, u1, v1, w1, u, v, w, mean ? mpart : nullptr,
+                           mean ? fmask : 0, fb, Corr{});
         return ok();
     });
 }

@@ -191,8 +191,10 @@ def test_fused_rhs_step_bitwise(orc, cuda, nx, ny, nz):
     import torch
     print(f"[fused rhs step] {nx}x{ny}x{nz}", flush=True)
     _, a, b = _step_vs_parts(orc, cuda, nx, ny, nz, (False, False, False))
-    for x, y in zip(a, b):
-        assert torch.equal(x, y)
+    # u, v, w and p; the RK1 stage (u1, v1, w1) is work space whose boundary
+    # fill (ghosts and w's wall row) the step leaves out
+    for q in (0, 1, 2, 6):
+        assert torch.equal(a[q], b[q]), q
 
 
 @pytest.mark.parametrize("is_forced", [(True, False, False), (True, True, False),
