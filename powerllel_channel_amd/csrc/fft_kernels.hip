@@ -524,9 +524,25 @@ struct SrcDivUnfilled {  // SrcDiv on a velocity whose boundary fill is not made
 };
 
 // Destinations of the x backward transform; (a, b) = normalised (x_2m, x_2m+1).
+// kAligned: the destination takes 16-byte aligned pairs (x_{2m-1}, x_{2m})
+// (row_pair) and single values (row_one) instead of (x_{2m}, x_{2m+1}) pairs,
+// whose haloed slots start 8 bytes into a 16-byte unit
 struct DstHalo64 {  // haloed FP64 array interior := value
+    static constexpr bool kAligned = true;
     double *p;
     long long off1, off2;
+    __device__ __forceinline__ void row_pair(long long j, long long k, int hi, double a,
+                                             double b) const
+    {
+        d2u pr;
+        pr.x = a;
+        pr.y = b;
+        *reinterpret_cast<d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + hi) = pr;
+    }
+    __device__ __forceinline__ void row_one(long long j, long long k, int hi, double a) const
+    {
+        p[(k + 1) * off1 + (j + 1) * off2 + hi] = a;
+    }
     __device__ __forceinline__ void store(long long, long long j, long long k, int m, int,
                                           double a, double b) const
     {
@@ -537,38 +553,47 @@ struct DstHalo64 {  // haloed FP64 array interior := value
     }
 };
 struct DstHaloFill {  // haloed FP64 interior := value, plus the boundary fill made from it
+    static constexpr bool kAligned = true;
     double *p;
     long long off1, off2;
     int nx, ny, nz;
     // updateBoundP at one rank: periodic images in x and y, zero-gradient
     // copies of the first/last plane into the ghost planes, corners included.
-    // A row's pair also goes to the row's images (other j ghost row, ghost
-    // planes), and the first/last value of every such row to its x ghosts.
-    __device__ __forceinline__ void row_put(long long rb, int i0, double a, double b) const
+    // Every value of a row also goes to the row's images (the other j ghost
+    // row, the ghost planes); x_0 and x_{nx-1} also to the row's x ghosts.
+    template <class Fn>
+    __device__ __forceinline__ void rows(long long j, long long k, Fn &&put) const
+    {
+        const int jj = (int)j + 1, kk = (int)k + 1;
+        const int jo = jj == ny ? 0 : (jj == 1 ? ny + 1 : -1);
+        put(kk * off1 + jj * off2);
+        if (jo >= 0) put(kk * off1 + jo * off2);
+        for (int side = 0; side < 2; ++side) {
+            if (side == 0 ? kk != 1 : kk != nz) continue;
+            const long long pg = (side == 0 ? 0 : nz + 1) * off1;
+            put(pg + jj * off2);
+            if (jo >= 0) put(pg + jo * off2);
+        }
+    }
+    __device__ __forceinline__ void row_pair(long long j, long long k, int hi, double a,
+                                             double b) const
     {
         d2u pr;
         pr.x = a;
         pr.y = b;
-        *reinterpret_cast<d2u *>(p + rb + i0) = pr;
-        if (i0 == 1) p[rb + nx + 1] = a;
-        if (i0 + 1 == nx) p[rb] = b;
+        rows(j, k, [&](long long rb) { *reinterpret_cast<d2u *>(p + rb + hi) = pr; });
     }
-    __device__ __forceinline__ void store(long long, long long j, long long k, int m, int,
-                                          double a, double b) const
+    __device__ __forceinline__ void row_one(long long j, long long k, int hi, double a) const
     {
-        const int i0 = 2 * m + 1, jj = (int)j + 1, kk = (int)k + 1;
-        const int jo = jj == ny ? 0 : (jj == 1 ? ny + 1 : -1);
-        row_put(kk * off1 + jj * off2, i0, a, b);
-        if (jo >= 0) row_put(kk * off1 + jo * off2, i0, a, b);
-        for (int side = 0; side < 2; ++side) {
-            if (side == 0 ? kk != 1 : kk != nz) continue;
-            const long long pg = (side == 0 ? 0 : nz + 1) * off1;
-            row_put(pg + jj * off2, i0, a, b);
-            if (jo >= 0) row_put(pg + jo * off2, i0, a, b);
-        }
+        // x_0 (hi = 1) is also the ghost at nx + 1; x_{nx-1} (hi = nx) the ghost at 0
+        rows(j, k, [&](long long rb) {
+            p[rb + hi] = a;
+            p[rb + (hi == 1 ? nx + 1 : 0)] = a;
+        });
     }
 };
 struct DstDenseSet {  // dense FP64 := value
+    static constexpr bool kAligned = false;
     double *P;
     __device__ __forceinline__ void store(long long row, long long, long long, int m, int nx,
                                           double a, double b) const
@@ -577,6 +602,7 @@ struct DstDenseSet {  // dense FP64 := value
     }
 };
 struct DstDenseAdd {  // dense FP64 += value
+    static constexpr bool kAligned = false;
     double *P;
     __device__ __forceinline__ void store(long long row, long long, long long, int m, int nx,
                                           double a, double b) const
@@ -587,6 +613,7 @@ struct DstDenseAdd {  // dense FP64 += value
     }
 };
 struct DstHaloFromDense {  // haloed FP64 interior := dense FP64 + value - eps
+    static constexpr bool kAligned = false;
     double *p;
     long long off1, off2;
     const double *P;
@@ -610,6 +637,7 @@ __device__ __forceinline__ double nmax(double m, double x)
     return (x > m || x != x) ? x : m;  // NaN wins
 }
 struct DstHaloAddN {  // haloed FP64 interior := (interior - eps) + value
+    static constexpr bool kAligned = false;
     double *p;
     long long off1, off2;
     const double *eps;  // gauge constant (device scalar), may be null
@@ -757,7 +785,28 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
     }
     __syncthreads();
     F::template run<+1, 2>(v, t, a.P, a.tw, ml);
-    if constexpr (!NORM) {
+    if constexpr (!NORM && Dst::kAligned && sizeof(S) == sizeof(double)) {
+        // x_{2m+1} of each pair through LDS to the next lane: stores of
+        // (x_{2m-1}, x_{2m}) at the even haloed slot 2m are 16-byte aligned;
+        // x_0 and x_{nx-1} go alone (FP64 rows: the row's LDS holds L doubles)
+        double *ls = reinterpret_cast<double *>(ml);
+        __syncthreads();  // the FFT's last LDS reads are done
+#pragma unroll
+        for (int e = 0; e < E; ++e) ls[t + e * T] = (double)v[e].y * a.norm;
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int m = t + e * T;
+                const double x0 = (double)v[e].x * a.norm;
+                if (m == 0)
+                    dst.row_one(j, k, 1, x0);
+                else
+                    dst.row_pair(j, k, 2 * m, ls[m - 1], x0);
+                if (m == L - 1) dst.row_one(j, k, 2 * L, ls[m]);
+            }
+        }
+    } else if constexpr (!NORM) {
         if (valid) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
