@@ -269,8 +269,9 @@ def main():
     ap.add_argument("--ppe-only", action="store_true", help="skip the RK2 step timing")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the whole "
-              "domain on one GPU (see DESIGN.md for how it would shard)", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus} is not supported: the whole domain runs on one "
+              "GPU (a z-slab split with one all-to-all each way around the z solve is the "
+              "way it would shard)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
