@@ -12,6 +12,9 @@ PPE solve.  Neither the body force nor the adjoint exists in the reference.
     win = WallShearWindow(channel, solver, nsteps=20, ckpt_every=5)
     J = wall_shear_objective(win, u0, v0, w0, bu, bv, bw)   # torch scalar
     J.backward()                                            # bu.grad, ...
+
+A window holds the state of its most recent forward only: backward of an
+objective whose window has run another forward since raises RuntimeError.
 """
 from __future__ import annotations
 
@@ -33,10 +36,14 @@ class WallShearWindow:
         check(lib().pll_adjoint_create(C.byref(h), channel._h, solver.handle, self.nsteps,
                                        self.ckpt_every))
         self._h = h
+        # The window's device state (checkpoints, body force) belongs to the
+        # most recent forward; backward differentiates that one only.
+        self.generation = 0
 
     def forward(self, u0, v0, w0, bu, bv, bw, stream=None) -> float:
         self.channel._chk(u0, v0, w0, bu, bv, bw)
         J = C.c_double()
+        self.generation += 1
         check(lib().pll_adjoint_forward(self._h, _p(u0), _p(v0), _p(w0), _p(bu), _p(bv), _p(bw),
                                         C.byref(J), _stream_ptr(stream)))
         return J.value
@@ -68,11 +75,17 @@ def _function():
         def forward(ctx, win, u0, v0, w0, bu, bv, bw):
             J = win.forward(u0, v0, w0, bu.contiguous(), bv.contiguous(), bw.contiguous())
             ctx.win = win
+            ctx.generation = win.generation
             return torch.tensor(J, dtype=torch.float64, device=bu.device)
 
         @staticmethod
         def backward(ctx, gJ):
             win = ctx.win
+            if win.generation != ctx.generation:
+                raise RuntimeError(
+                    "WallShearWindow holds one live graph: another forward ran on this "
+                    "window after the one being differentiated (its checkpoints and body "
+                    "force were overwritten); use one window per live objective")
             shape = win.channel.shape
             dev = gJ.device
             g = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)]

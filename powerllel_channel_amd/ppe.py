@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import PLLError, check, lib
 
 
 def _ptr_and_kind(a):
@@ -187,11 +187,25 @@ def init_poisson_solver(nx_global, ny_global, nz_global, dx, dy, dzf_global,
                                            float(dx), float(dy), dzf.ctypes.data,
                                            _bc(bctype_x), _bc(bctype_y), _bc(bctype_z),
                                            None if nb is None else nb.ctypes.data))
+    global _DROPIN_SHAPE
+    _DROPIN_SHAPE = (int(nz_global) + 2, int(ny_global) + 2, int(nx_global) + 2)
+
+
+# haloed shape p(0:nx+1, 0:ny+1, 0:nz+1) of the initialised drop-in instance
+_DROPIN_SHAPE = None
 
 
 def execute_poisson_solver(var):
     """Mirror of execute_poisson_solver (mod_poisson_solver.c:468): in place on
-    a host numpy array or a device torch tensor; synchronous."""
+    a host numpy array or a device torch tensor; synchronous.  The C entry
+    point takes a bare pointer (as the reference's does) and touches
+    (nx+2)(ny+2)(nz+2) doubles behind it, so the shape is checked here
+    against the one init_poisson_solver was given."""
+    if _DROPIN_SHAPE is None:
+        raise PLLError(-4, "execute_poisson_solver before init_poisson_solver")
+    if tuple(var.shape) != _DROPIN_SHAPE:
+        raise ValueError(f"var has shape {tuple(var.shape)}, expected {_DROPIN_SHAPE} "
+                         "(nz+2, ny+2, nx+2)")
     addr, _dev = _ptr_and_kind(var)
     check(lib().execute_poisson_solver_status(addr))
 
@@ -203,4 +217,6 @@ def set_poisson_solver_refinement(sweeps: int):
 
 
 def free_poisson_solver():
+    global _DROPIN_SHAPE
     check(lib().free_poisson_solver_status())
+    _DROPIN_SHAPE = None

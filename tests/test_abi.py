@@ -53,7 +53,7 @@ def _create(lib, nx, ny, nz, bcx=b"PP", bcy=b"PP", bcz=b"NN", dzf=None):
     ((7, 8, 8), -1),              # odd nx
     ((8, 7, 8), -1),              # odd ny
     ((8, 8, 1), -1),              # nz < 2
-    ((10, 8, 8), -5),             # nx/2 = 5: radix 5 not implemented
+    ((22, 8, 8), -5),             # nx/2 = 11: prime factor above 7
 ])
 def test_create_rejects_bad_sizes(lib, args, code):
     rc, h = _create(lib, *args)
@@ -101,3 +101,54 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_dropin_python_checks_shape(monkeypatch):
+    """The C entry point takes a bare pointer, as the reference's does; the
+    Python mirror rejects an array whose shape is not the initialised
+    (nz+2, ny+2, nx+2) before any call, and any call before init."""
+    from powerllel_channel_amd import ppe
+    from powerllel_channel_amd._lib import PLLError
+    monkeypatch.setattr(ppe, "_DROPIN_SHAPE", None)
+    with pytest.raises(PLLError):
+        ppe.execute_poisson_solver(np.zeros((10, 10, 10)))
+    monkeypatch.setattr(ppe, "_DROPIN_SHAPE", (10, 8, 18))
+    for bad in ((10, 8, 17), (8, 10, 18), (10 * 8 * 18,)):
+        with pytest.raises(ValueError):
+            ppe.execute_poisson_solver(np.zeros(bad))
+
+
+def test_adjoint_window_single_live_graph():
+    """Two objectives on one window: the first one's backward must refuse (the
+    window's checkpoints and body force belong to the second forward); the
+    second one differentiates normally.  A stand-in window records calls so
+    that no GPU is involved."""
+    import torch
+    from powerllel_channel_amd import adjoint
+
+    class Win:
+        def __init__(self):
+            self.generation, self.calls = 0, []
+            self.channel = type("Ch", (), {"shape": (3, 4, 5)})()
+
+        def forward(self, u0, v0, w0, bu, bv, bw):
+            self.generation += 1
+            self.calls.append("f")
+            return float(bu.sum())
+
+        def backward(self, gu, gv, gw):
+            self.calls.append("b")
+            for g in (gu, gv, gw):
+                g.fill_(1.0)
+
+    win = Win()
+    z = torch.zeros(3, 4, 5, dtype=torch.float64)
+    b1 = [torch.zeros(3, 4, 5, dtype=torch.float64, requires_grad=True) for _ in range(3)]
+    b2 = [torch.ones(3, 4, 5, dtype=torch.float64, requires_grad=True) for _ in range(3)]
+    J1 = adjoint.wall_shear_objective(win, z, z, z, *b1)
+    J2 = adjoint.wall_shear_objective(win, z, z, z, *b2)
+    with pytest.raises(RuntimeError, match="one live graph"):
+        J1.backward()
+    J2.backward()
+    assert win.calls == ["f", "f", "b"]
+    assert torch.equal(b2[0].grad, torch.ones(3, 4, 5, dtype=torch.float64))

@@ -4,9 +4,12 @@ Poiseuille start, 10 RK2 steps; FP64 PPE against the FP32 + refinement PPE at
 a 1e-11 pressure tolerance; a 5-step adjoint window with checkpoints.
 
 The oracle cannot step 604M cells in test time, so the checks here are the
-size-independent ones (the same kernels are pinned to the oracle at smaller
-sizes, and the transform lengths of this grid, 768 and 1024, and the z length
-384 are covered by the PPE parity cases):
+size-independent ones.  The kernels this grid runs are pinned to the oracle
+elsewhere at the same lengths and layouts: the x and y transform lengths 768
+and 1024 by the PPE parity case 1536x1024x8, the z length 384 (12 rows per
+lane, 32 lanes per column, 16-column tiles) by the cases 64x48x384 and
+32x36x384 of tests/test_gpu_ppe.py, and the step kernels at 768x384x256 by
+tests/test_gpu_step.py::test_steps_full_size_vs_oracle:
 * after every projection step, the bulk velocity equals the forcing target
   (forceVel, src/mod_calcVel.f90:455-509), and max |div u| < 1e-12 (FP64)
   from step 2 on.  Step 1 projects the white-noise start, whose divergence

@@ -22,6 +22,13 @@ CASES = [
     (128, 96, 32, 4.4),
     (192, 192, 128, 3.0),
     (256, 192, 128, 3.0),
+    # z lengths of the two large grids with the z-solve layout chosen there:
+    # nz = 256 -> 16 rows per lane, 16 lanes per column (DPP row exchanges),
+    # 32-column tiles; nz = 384 -> 12 rows per lane, 32 lanes per column
+    # (ds_bpermute exchanges and lane scans), 16-column tiles
+    (64, 48, 256, 4.0),
+    (64, 48, 384, 4.4),
+    (32, 36, 384, 4.0),
     # transform lengths of the 1536x1024x384 grid: x 768 (nx/2), y 1024; y 768
     (1536, 1024, 8, 4.4),
     (64, 768, 16, 4.4),
@@ -78,6 +85,31 @@ def test_ppe_matches_oracle(orc, cuda, nx, ny, nz, sr):
     assert e_mr <= TOL
     assert e_raw <= TOL
     assert e_lit <= TOL
+
+
+def test_ppe_full_size_vs_oracle(orc, cuda):
+    """768x384x256 (precision workload, stretch 4): the FP64 solve against the
+    oracle on the full grid, raw (same pinned gauge) and mean-removed."""
+    import torch
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, mean_removed, spectral_rhs
+    nx, ny, nz = 768, 384, 256
+    print(f"[ppe parity full size] {nx}x{ny}x{nz}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 4.0)
+    rhs = spectral_rhs(g, 3)
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    p = torch.from_numpy(rhs).to(cuda)
+    s.solve(p)
+    torch.cuda.synchronize()
+    orc.OraclePPE(nx, ny, nz, g.dx, g.dy, g.dzf, pin_gauge=True).solve(rhs)
+    out = p.cpu().numpy()
+    del p
+    inner = (slice(1, -1),) * 3
+    e_raw = _rel(out[inner], rhs[inner])
+    e_mr = _rel(mean_removed(g, out), mean_removed(g, rhs))
+    print(f"  raw(pinned) {e_raw:.2e} mean-removed {e_mr:.2e}")
+    assert e_raw <= TOL and e_mr <= TOL
 
 
 def test_ppe_residual_full_size(cuda):

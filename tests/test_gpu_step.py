@@ -132,9 +132,8 @@ def _run_steps(orc, cuda, case, nsteps):
         torch.cuda.synchronize()
         div_n, _ = ch.max_div(du, dv, dw)
         worst = max(worst, div_n)
-        if nsteps <= 10 or (n + 1) % 10 == 0:
-            print(f"  step {n + 1:3d}: {e0.elapsed_time(e1):6.3f} ms (PPE {sum(s.stage_times()):6.3f} ms)"
-                  f"  max|div| {div_n:.2e}", flush=True)
+        print(f"  step {n + 1:3d}: {e0.elapsed_time(e1):6.3f} ms (PPE {sum(s.stage_times()):6.3f} ms)"
+              f"  max|div| {div_n:.2e}", flush=True)
     s.set_timing(False)
     return _host(du, dv, dw), (u, v, w), worst
 
@@ -150,6 +149,44 @@ def test_one_step_64(orc, cuda):
         assert rel <= 1e-12
     print(f"  max div {div:.2e}")
     assert div < 1e-12
+
+
+def test_steps_full_size_vs_oracle(orc, cuda):
+    """768x384x256 (stretch 4, nu = 1/5600, dt = 1e-4, the step workload of the
+    benchmark): three steps through pll_channel_run, whose projection is
+    carried into the next step's RK1 on a grid this size, against three
+    oracle steps; velocity and pressure within 1e-12, max |div u| < 1e-12."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, mean_removed, poiseuille_fields
+    nx, ny, nz, nu, dt, nsteps = 768, 384, 256, 1 / 5600, 1e-4, 3
+    print(f"[steps full size] {nx}x{ny}x{nz}, {nsteps} steps via run()", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 4.0)
+    G = orc.OracleGrid(nx, ny, nz, LX, LY, LZ, g.dzf, nu, dt)
+    u, v, w = poiseuille_fields(g, 17)
+    orc.bc_vel(G, u, v, w, 0.0)
+    u1, v1, w1, p = [np.zeros(g.shape) for _ in range(4)]
+    d = _dev(torch, cuda, u, v, w, u1, v1, w1, p)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, nu, dt)
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    ch.run(s, nsteps, *d)
+    ppe = orc.OraclePPE(nx, ny, nz, g.dx, g.dy, g.dzf, pin_gauge=True)
+    for _ in range(nsteps):
+        orc.step(G, ppe, u, v, w, u1, v1, w1, p)
+    torch.cuda.synchronize()
+    div, _ = ch.max_div(d[0], d[1], d[2])
+    for name, t, ref in zip("uvw", d[:3], (u, v, w)):
+        rel = float(np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max())
+        print(f"  {name} rel {rel:.2e}", flush=True)
+        assert rel <= 1e-12, (name, rel)
+    gp = d[6].cpu().numpy()
+    ep = float(np.abs(mean_removed(g, gp) - mean_removed(g, p)).max()
+               / np.abs(mean_removed(g, p)).max())
+    print(f"  p rel (mean-removed) {ep:.2e}, max|div u| {div:.2e}")
+    assert ep <= 1e-12 and div < 1e-12
+    s.close()
+    ch.close()
 
 
 def _step_vs_parts(orc, cuda, nx, ny, nz, is_forced):
