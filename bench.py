@@ -217,10 +217,22 @@ def copy_bandwidth(torch, dev):
     return 2 * n * 8 / dt / 1e9
 
 
-def cpu_baseline(ppe_solves=5, steps=2):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ppe_solves=12, steps=10, warmup=2):
     """The oracle (C/OpenMP restatement of the reference CPU algorithm, in-repo
-    FFT in place of FFTW) on the host cores: PPE solves and RK2 steps at
-    768x384x256 after one warm-up each."""
+    FFT in place of FFTW) on the host cores: the median of `ppe_solves` PPE
+    solves and of `steps` RK2 steps at 768x384x256 after `warmup` of each,
+    with per-stage medians under the reference's GPTL bucket names
+    (src/src_c/mod_poisson_solver.c:480-602, src/main.f90:131-212)."""
     from oracle import oracle as O
     from powerllel_channel_amd.mesh import Grid
     from powerllel_channel_amd.workloads import LX, LY, LZ, poiseuille_fields, spectral_rhs
@@ -229,32 +241,46 @@ def cpu_baseline(ppe_solves=5, steps=2):
     g = Grid(NX, NY, NZ, LX, LY, LZ, 2.0 * GAMMA)
     p = spectral_rhs(g, 0)
     ppe = O.OraclePPE(NX, NY, NZ, g.dx, g.dy, g.dzf, pin_gauge=True)
-    ppe.solve(p)
-    t0 = time.perf_counter()
-    for _ in range(ppe_solves):
+    t_solve, t_stage = [], []
+    for it in range(warmup + ppe_solves):
+        t0 = time.perf_counter()
         ppe.solve(p)
-    t_ppe = (time.perf_counter() - t0) / ppe_solves
-    log(f"[bench] oracle PPE {t_ppe:.3f} s/solve on {threads} threads")
+        dt_ = time.perf_counter() - t0
+        if it >= warmup:
+            t_solve.append(dt_)
+            t_stage.append(ppe.stage_times())
+    t_ppe = float(np.median(t_solve))
+    ppe_stages = dict(zip(O.PPE_BUCKETS, np.median(np.array(t_stage), axis=0).tolist()))
+    log(f"[bench] oracle PPE {t_ppe:.3f} s/solve (median of {ppe_solves}) on {threads} threads")
     G = O.OracleGrid(NX, NY, NZ, LX, LY, LZ, g.dzf, 1.0 / 5600, 1e-4)
     u, v, w = poiseuille_fields(g, 0)
     u1, v1, w1 = (np.zeros(g.shape) for _ in range(3))
     O.bc_vel(G, u, v, w, 0.0)
-    O.step(G, ppe, u, v, w, u1, v1, w1, p)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        O.step(G, ppe, u, v, w, u1, v1, w1, p)
-    t_step = (time.perf_counter() - t0) / steps
-    log(f"[bench] oracle RK2 step {t_step:.3f} s/step")
+    t_steps, buckets = [], []
+    for it in range(warmup + steps):
+        t0 = time.perf_counter()
+        b = O.step_timed(G, ppe, u, v, w, u1, v1, w1, p)
+        dt_ = time.perf_counter() - t0
+        if it >= warmup:
+            t_steps.append(dt_)
+            buckets.append([b[k] for k in O.STEP_BUCKETS])
+    t_step = float(np.median(t_steps))
+    step_stages = dict(zip(O.STEP_BUCKETS, np.median(np.array(buckets), axis=0).tolist()))
+    log(f"[bench] oracle RK2 step {t_step:.3f} s/step (median of {steps})")
     ppe.close()
     return {
-        "value": 1.0 / t_ppe,
+        "value": round(1.0 / t_ppe, 3),
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{ppe_solves} FP64 PPE solves and {steps} RK2 step(s) at {NX}x{NY}x{NZ} "
-                   "after one warm-up each; oracle = C/OpenMP restatement of the reference "
-                   "CPU algorithm with an in-repo FFT in place of FFTW"),
-        "rk2_steps_per_s": 1.0 / t_step,
+        "cpu_model": cpu_model(),
+        "sample": (f"median of {ppe_solves} FP64 PPE solves and of {steps} RK2 steps at "
+                   f"{NX}x{NY}x{NZ} after {warmup} warm-ups each, {threads} OpenMP threads; "
+                   "oracle = C/OpenMP restatement of the reference CPU algorithm with an "
+                   "in-repo FFT in place of FFTW"),
+        "rk2_steps_per_s": round(1.0 / t_step, 3),
+        "ppe_stage_s": {k: round(v, 5) for k, v in ppe_stages.items()},
+        "rk2_stage_s": {k: round(v, 5) for k, v in step_stages.items()},
     }
 
 

@@ -966,6 +966,40 @@ void orc_step_body(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, do
     orc_bc_vel(g, u, v, w, u_crf);
 }
 
+/* orc_step with the wall time of each of the reference's GPTL buckets of the
+ * main loop (src/main.f90:131-212): t = {uvw1, Update boundary vel (three
+ * calls summed), uvw2, Calculate RHS, Poisson solver, Update boundary pres,
+ * Correct vel, Force vel}, in seconds. */
+void orc_step_timed(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
+                    double *u1, double *v1, double *w1, double *p, double u_crf,
+                    const int is_forced[3], const double vel_force[3], double t[8])
+{
+    double c0 = now_s(), c1;
+    for (int i = 0; i < 8; i++) t[i] = 0.0;
+#define LAP(slot) (c1 = now_s(), t[slot] += c1 - c0, c0 = c1)
+    orc_rk1(g, u, v, w, u1, v1, w1, u_crf);
+    LAP(0);
+    orc_bc_vel(g, u1, v1, w1, u_crf);
+    LAP(1);
+    orc_rk2(g, u, v, w, u1, v1, w1, u_crf);
+    LAP(2);
+    orc_bc_vel(g, u, v, w, u_crf);
+    LAP(1);
+    orc_calc_rhs(g, u, v, w, p);
+    LAP(3);
+    orc_ppe_solve(ppe, p);
+    LAP(4);
+    orc_bc_p(g, p);
+    LAP(5);
+    orc_correct_vel(g, p, u, v, w);
+    LAP(6);
+    orc_force_vel(g, is_forced, vel_force, u, v, w);
+    LAP(7);
+    orc_bc_vel(g, u, v, w, u_crf);
+    LAP(1);
+#undef LAP
+}
+
 void orc_step(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
               double *u1, double *v1, double *w1, double *p, double u_crf,
               const int is_forced[3], const double vel_force[3])

@@ -112,6 +112,12 @@ void orc_step_body(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, do
                    const int is_forced[3], const double vel_force[3], const double *bu,
                    const double *bv, const double *bw);
 /* One full time step in the order of src/main.f90:137-208. */
+/* orc_step timed per GPTL bucket of src/main.f90:131-212 (seconds): uvw1,
+ * Update boundary vel, uvw2, Calculate RHS, Poisson solver, Update boundary
+ * pres, Correct vel, Force vel. */
+void orc_step_timed(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
+                    double *u1, double *v1, double *w1, double *p, double u_crf,
+                    const int is_forced[3], const double vel_force[3], double t[8]);
 void orc_step(const orc_grid_t *g, orc_ppe_t *ppe, double *u, double *v, double *w,
               double *u1, double *v1, double *w1, double *p, double u_crf,
               const int is_forced[3], const double vel_force[3]);

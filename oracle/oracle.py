@@ -81,6 +81,8 @@ def lib():
         "orc_has_nan": (C.c_int, [C.c_long, _dp]),
         "orc_step": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                             C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "orc_step_timed": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_double), _dp]),
         "orc_step_body": (None, [gp, vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double), _dp, _dp, _dp]),
         "orc_set_threads": (None, [C.c_int]),
@@ -280,6 +282,23 @@ def step(G, ppe: OraclePPE, u, v, w, u1, v1, w1, p, u_crf=0.0,
          is_forced=(True, False, False), vel_force=(1.0, 0.0, 0.0)):
     lib().orc_step(G.ref(), ppe.h, u, v, w, u1, v1, w1, p, u_crf, _iv(is_forced),
                    _fv(vel_force))
+
+
+# the reference's GPTL bucket names (src/main.f90:131-212 and
+# src/src_c/mod_poisson_solver.c:480-602) for the oracle's timings
+STEP_BUCKETS = ("uvw1", "Update boundary vel", "uvw2", "Calculate RHS", "Poisson solver",
+                "Update boundary pres", "Correct vel", "Force vel")
+PPE_BUCKETS = ("--Copy & Fwd X-FFT", "--Forward Y-FFT", "--Solve trid", "--Backward Y-FFT",
+               "--Bwd X-FFT & Copy")
+
+
+def step_timed(G, ppe: OraclePPE, u, v, w, u1, v1, w1, p, u_crf=0.0,
+               is_forced=(True, False, False), vel_force=(1.0, 0.0, 0.0)):
+    """step() returning {GPTL bucket: seconds} for this step."""
+    t = np.zeros(8)
+    lib().orc_step_timed(G.ref(), ppe.h, u, v, w, u1, v1, w1, p, u_crf, _iv(is_forced),
+                         _fv(vel_force), t)
+    return dict(zip(STEP_BUCKETS, t.tolist()))
 
 
 def step_body(G, ppe: OraclePPE, u, v, w, u1, v1, w1, p, bu, bv, bw, u_crf=0.0,
