@@ -33,7 +33,7 @@ HIP_SOURCES = {
     # restates the reference's Fortran arithmetic: no fused multiply-add
     "step_kernels.hip": ["-ffp-contract=off"],
 }
-C_SOURCES = ["ppe.c", "channel.c", "adjoint.c"]
+C_SOURCES = ["ppe.c", "channel.c", "adjoint.c", "hostcopy.c"]
 
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
@@ -101,7 +101,7 @@ def build(verbose: bool = False) -> str:
                 print(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp,
-          f"-L{ROCM}/lib", "-lamdhip64", "-lm"])
+          f"-L{ROCM}/lib", "-lamdhip64", "-lm", "-lpthread"])
     os.replace(tmp, LIB)
     if build_io() is None and verbose:
         print(f"no HDF5 under {HDF5}: libpllio.so (field files) not built")

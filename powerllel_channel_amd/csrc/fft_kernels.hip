@@ -172,6 +172,88 @@ struct Dft<8, DIR> {
     }
 };
 
+// Odd prime radices R = 5, 7, 11, 13: cos(2 pi p/R) and sin(2 pi p/R) for
+// p = 1..(R-1)/2.
+template <int R>
+struct OddRoots;
+template <>
+struct OddRoots<5> {
+    static constexpr double c[2] = {0.30901699437494742410, -0.80901699437494742410};
+    static constexpr double s[2] = {0.95105651629515357212, 0.58778525229247312917};
+};
+template <>
+struct OddRoots<7> {
+    static constexpr double c[3] = {0.62348980185873353053, -0.22252093395631440429,
+                                    -0.90096886790241912624};
+    static constexpr double s[3] = {0.78183148246802980871, 0.97492791218182360702,
+                                    0.43388373911755812048};
+};
+template <>
+struct OddRoots<11> {
+    static constexpr double c[5] = {0.84125353283118116886, 0.41541501300188642553,
+                                    -0.14231483827328514044, -0.65486073394528506406,
+                                    -0.95949297361449738989};
+    static constexpr double s[5] = {0.54064081745559758211, 0.90963199535451837141,
+                                    0.98982144188093273238, 0.75574957435425828377,
+                                    0.28173255684142969771};
+};
+template <>
+struct OddRoots<13> {
+    static constexpr double c[6] = {0.88545602565320989590, 0.56806474673115580251,
+                                    0.12053668025532305335, -0.35460488704253562597,
+                                    -0.74851074817110109863, -0.97094181742605202716};
+    static constexpr double s[6] = {0.46472317204376854566, 0.82298386589365639458,
+                                    0.99270887409805399280, 0.93501624268541482344,
+                                    0.66312265824079520238, 0.23931566428755776715};
+};
+
+// Odd prime R: X_m = x_0 + sum_{j=1..H} (cos(2 pi jm/R) (x_j + x_{R-j})
+//                        + DIR i sin(2 pi jm/R) (x_j - x_{R-j})), H = (R-1)/2,
+// and X_{R-m} the same with the sine part negated.
+template <int R, int DIR>
+struct DftOdd {
+    template <typename V>
+    static __device__ __forceinline__ void run(V *x)
+    {
+        using S = typename Sc<V>::T;
+        constexpr int H = (R - 1) / 2;
+        V tp[H], dm[H];
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            tp[j - 1] = cadd(x[j], x[R - j]);
+            dm[j - 1] = csub(x[j], x[R - j]);
+        }
+        const V x0 = x[0];
+        V sum = x0;
+#pragma unroll
+        for (int j = 0; j < H; ++j) sum = cadd(sum, tp[j]);
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            V a = x0, b = mk<V>(S(0), S(0));
+#pragma unroll
+            for (int j = 1; j <= H; ++j) {
+                const int p = (j * m) % R;
+                const S cv = S(p <= H ? OddRoots<R>::c[p - 1] : OddRoots<R>::c[R - p - 1]);
+                const S sv = S(p <= H ? OddRoots<R>::s[p - 1] : -OddRoots<R>::s[R - p - 1]);
+                a = mk<V>(fma_(cv, tp[j - 1].x, a.x), fma_(cv, tp[j - 1].y, a.y));
+                b = mk<V>(fma_(sv, dm[j - 1].x, b.x), fma_(sv, dm[j - 1].y, b.y));
+            }
+            const V rb = rot<DIR>(b);
+            x[m] = cadd(a, rb);
+            x[R - m] = csub(a, rb);
+        }
+        x[0] = sum;
+    }
+};
+template <int DIR>
+struct Dft<5, DIR> : DftOdd<5, DIR> {};
+template <int DIR>
+struct Dft<7, DIR> : DftOdd<7, DIR> {};
+template <int DIR>
+struct Dft<11, DIR> : DftOdd<11, DIR> {};
+template <int DIR>
+struct Dft<13, DIR> : DftOdd<13, DIR> {};
+
 // One Stockham radix-R stage.  Thread t holds elements t + e*T (e < E).
 // Butterfly b = t + q*T (q < E/R) takes elements b + r*L/R = slot q + r*E/R,
 // twiddles them by w^(r*k) (k = b mod Ns, w = exp(DIR 2 pi i/(Ns R))) and
@@ -211,6 +293,80 @@ __device__ __forceinline__ void stage(V (&v)[E], int t, int T, int Ns, int twste
     }
 }
 
+// The same stage when R does not divide E: the row is in LDS in natural
+// element order; thread t takes butterflies b = t + qT < L/R (at most
+// ceil(E/R) of them), reads their R inputs each, and after a barrier writes
+// the outputs back to LDS at the Stockham positions.
+template <int R, int E, int DIR, typename V>
+__device__ __forceinline__ void stage_lds(int t, int T, int L, int Ns, int twstep,
+                                          const V *__restrict__ tw, V *lds)
+{
+    constexpr int QM = (E + R - 1) / R;
+    const int nb = L / R;
+    V x[QM][R];
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+        const int b = t + q * T;
+        if (b < nb) {
+            const int k = b % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[q][r] = lds[b + r * nb];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    V w = tw[r * k * twstep];
+                    if (DIR > 0) w.y = -w.y;
+                    x[q][r] = cmul(x[q][r], w);
+                }
+            }
+            Dft<R, DIR>::run(x[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+        const int b = t + q * T;
+        if (b < nb) {
+            const int k = b % Ns;
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[base + r * Ns] = x[q][r];
+        }
+    }
+}
+
+// Stage s of a runtime plan.  from_lds: the previous stage left the row in
+// LDS (every stage but the first).  A radix dividing E runs from registers;
+// any other radix runs from LDS, and as the last stage brings the result
+// back into registers.
+template <int R, int E, int DIR, typename V>
+__device__ __forceinline__ void stage_rt(V (&v)[E], int t, int T, int L, int Ns, int twstep,
+                                         const V *__restrict__ tw, V *lds, bool from_lds,
+                                         bool last)
+{
+    if constexpr (E % R == 0) {
+        if (from_lds) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[t + e * T];
+            __syncthreads();
+        }
+        stage<R, E, DIR>(v, t, T, Ns, twstep, tw, lds, last);
+    } else {
+        if (!from_lds) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) lds[t + e * T] = v[e];
+        }
+        __syncthreads();
+        stage_lds<R, E, DIR>(t, T, L, Ns, twstep, tw, lds);
+        if (last) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[t + e * T];
+        }
+    }
+}
+
 // Full transform with a runtime plan.  On entry v holds elements t + e*T of
 // the input; on exit it holds elements t + e*T of the output.  `lds` is this
 // transform's LDS segment (>= L points).  twscale = (table length) / L.
@@ -223,20 +379,18 @@ __device__ __forceinline__ void fft_run(V (&v)[E], int t, const pll_fftplan &P,
     int Ns = 1;
     for (int s = 0; s < P.nst; ++s) {
         const int R = P.rad[s];
-        const bool last = (s == P.nst - 1);
-        if (s > 0) {
-            __syncthreads();
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = lds[t + e * T];
-            __syncthreads();
-        }
+        const bool last = (s == P.nst - 1), fl = s > 0;
         const int twstep = twscale * (L / (Ns * R));
         switch (R) {
-            case 2: stage<2, E, DIR>(v, t, T, Ns, twstep, tw, lds, last); break;
-            case 3: stage<3, E, DIR>(v, t, T, Ns, twstep, tw, lds, last); break;
-            case 4: stage<4, E, DIR>(v, t, T, Ns, twstep, tw, lds, last); break;
-            case 6: stage<6, E, DIR>(v, t, T, Ns, twstep, tw, lds, last); break;
-            case 8: stage<8, E, DIR>(v, t, T, Ns, twstep, tw, lds, last); break;
+            case 2: stage_rt<2, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 3: stage_rt<3, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 4: stage_rt<4, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 5: stage_rt<5, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 6: stage_rt<6, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 7: stage_rt<7, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 8: stage_rt<8, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 11: stage_rt<11, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
+            case 13: stage_rt<13, E, DIR>(v, t, T, L, Ns, twstep, tw, lds, fl, last); break;
             default: break;
         }
         Ns *= R;
@@ -659,7 +813,9 @@ template <typename V>
 struct XArgs {
     pll_fftplan P;
     const V *tw;        // [nx] exp(-2 pi i t/nx)
-    long long nrows;    // ny*nz
+    long long row0;     // first row of this launch (0: the whole array)
+    int xcd;            // number workgroups per XCD (neighbouring rows share an L2)
+    long long nrows;    // one past its last row (ny*nz: the whole array)
     int nx, ny, rows_per_wg, pitch;
     double norm;
     double *norm_part;  // NORM kernels: [2 * workgroups] (max |value|, max |result|)
@@ -682,15 +838,18 @@ struct YArgs {
     int L, ny, cols, ntiles, pitch;
 };
 
-template <int E>
+// FP64 kernels with 12 or more points per thread need more than 128 VGPRs,
+// so their workgroups are capped at 256 threads; FP32 values take half the
+// registers.
+template <int E, typename V = double2>
 constexpr int max_threads()
 {
-    return E >= 12 ? 256 : 512;
+    return (E >= 12 && sizeof(V) == sizeof(double2)) ? 256 : 512;
 }
-template <class F>
+template <class F, typename V = double2>
 constexpr int max_threads_f()
 {
-    return max_threads<F::E>();
+    return max_threads<F::E, V>();
 }
 
 // x forward: real rows -> packed half spectrum rows of W.
@@ -703,7 +862,7 @@ __device__ __forceinline__ V tw_m(const V *__restrict__ tw, V wt, int e, int T)
 }
 
 template <class F, typename V, class Src>
-__global__ void __launch_bounds__(max_threads_f<F>())
+__global__ void __launch_bounds__((max_threads_f<F, V>()))
 k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
 {
     using S = typename Sc<V>::T;
@@ -711,7 +870,7 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
-    const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
+    const long long row = a.row0 + (long long)(a.xcd ? xcd_block() : blockIdx.x) * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
     unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
@@ -748,7 +907,7 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
 // x backward: packed half spectrum rows -> real rows * norm, stored by Dst;
 // with NORM, also the workgroup's max |value| and max |result|.
 template <class F, typename V, class Dst, bool NORM = false>
-__global__ void __launch_bounds__(max_threads_f<F>())
+__global__ void __launch_bounds__((max_threads_f<F, V>()))
 k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
 {
     using S = typename Sc<V>::T;
@@ -756,7 +915,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
-    const long long row = (long long)blockIdx.x * a.rows_per_wg + rl;
+    const long long row = a.row0 + (long long)(a.xcd ? xcd_block() : blockIdx.x) * a.rows_per_wg + rl;
     const bool valid = row < a.nrows;
     unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
@@ -878,7 +1037,7 @@ __global__ void __launch_bounds__(256) k_norm_final(const double *__restrict__ p
 
 // y forward, in place on W: tile of `cols` adjacent columns of one k-plane.
 template <class F, typename V>
-__global__ void __launch_bounds__(max_threads_f<F>())
+__global__ void __launch_bounds__((max_threads_f<F, V>()))
 k_yfwd(V *__restrict__ W, YArgs<V> a)
 {
     using S = typename Sc<V>::T;
@@ -927,7 +1086,7 @@ k_yfwd(V *__restrict__ W, YArgs<V> a)
 }
 
 template <class F, typename V>
-__global__ void __launch_bounds__(max_threads_f<F>())
+__global__ void __launch_bounds__((max_threads_f<F, V>()))
 k_ybwd(V *__restrict__ W, YArgs<V> a)
 {
     using S = typename Sc<V>::T;
@@ -982,6 +1141,8 @@ XArgs<V> x_args(const pll_ppe_dev *d, const void *tw)
     XArgs<V> a;
     a.P = d->px;
     a.tw = (const V *)tw;
+    a.row0 = 0;
+    a.xcd = 0;
     a.nrows = (long long)d->ny * d->nz;
     a.nx = d->nx;
     a.ny = d->ny;
@@ -991,14 +1152,22 @@ XArgs<V> x_args(const pll_ppe_dev *d, const void *tw)
     return a;
 }
 
+// Rows [row0, row1) of the ny*nz rows (row = k ny + j); row1 < 0: all rows.
 template <class F, typename V, class Src>
-int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_t s)
+int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_t s,
+              long long row0 = 0, long long row1 = -1)
 {
-    const XArgs<V> a = x_args<V>(d, tw);
+    XArgs<V> a = x_args<V>(d, tw);
+    if (row1 >= 0) {
+        if (row0 < 0 || row0 >= row1 || row1 > a.nrows) return -1;
+        a.row0 = row0;
+        a.nrows = row1;
+    }
     const int threads = a.rows_per_wg * a.P.T;
-    const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
+    const long long blocks = (a.nrows - a.row0 + a.rows_per_wg - 1) / a.rows_per_wg;
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
-    if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    if (threads > max_threads_f<F, V>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    a.xcd = d->x_xcd >= 2 || (d->x_xcd == 1 && sizeof(V) == sizeof(float2));
     hipLaunchKernelGGL((k_xfwd<F, V, Src>), dim3((unsigned)blocks), dim3(threads), shm, s, src,
                        (V *)W, a.tw, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -1006,13 +1175,19 @@ int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_
 
 template <class F, typename V, class Dst, bool NORM = false>
 int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipStream_t s,
-              double *norm = nullptr)
+              double *norm = nullptr, long long row0 = 0, long long row1 = -1)
 {
     XArgs<V> a = x_args<V>(d, tw);
+    if (row1 >= 0) {
+        if (NORM || row0 < 0 || row0 >= row1 || row1 > a.nrows) return -1;
+        a.row0 = row0;
+        a.nrows = row1;
+    }
     const int threads = a.rows_per_wg * a.P.T;
-    const long long blocks = (a.nrows + a.rows_per_wg - 1) / a.rows_per_wg;
+    const long long blocks = (a.nrows - a.row0 + a.rows_per_wg - 1) / a.rows_per_wg;
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
-    if (threads > max_threads_f<F>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    if (threads > max_threads_f<F, V>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
+    a.xcd = d->x_xcd >= 2 || (d->x_xcd == 1 && sizeof(V) == sizeof(float2));
     if (NORM) {
         // norm[0..1]: result; norm[2 ..): partials, room for ny*nz workgroups
         if (!norm || blocks > a.nrows || shm < sizeof(double) * 2 * ((threads + 63) / 64))
@@ -1038,13 +1213,13 @@ int launch_y(bool fwd, const pll_ppe_dev *d, const void *tw, void *W, hipStream_
     a.tw = (const V *)tw;
     a.L = d->L;
     a.ny = d->ny;
-    a.cols = d->y_cols_per_wg;
+    a.cols = sizeof(V) == sizeof(double2) ? d->y_cols_per_wg : d->y_cols32_per_wg;
     a.ntiles = (d->L + a.cols - 1) / a.cols;
     a.pitch = d->ny + 1;
     const int threads = a.cols * a.P.T;
     const long long blocks = (long long)a.ntiles * d->nz;
     const size_t shm = (size_t)a.cols * a.pitch * lds_elem<F, V>();
-    if (threads > max_threads_f<F>() || shm > 163840 || blocks > 0x7fffffffLL) return -1;
+    if (threads > max_threads_f<F, V>() || shm > 163840 || blocks > 0x7fffffffLL) return -1;
     // long columns (ny = 1024 in FP64: 16 KB each) need more than the default
     // 64 KB of LDS for a tile of 4 columns
     const void *fn = fwd ? (const void *)k_yfwd<F, V> : (const void *)k_ybwd<F, V>;
@@ -1089,7 +1264,9 @@ int dispatch_plan(const pll_fftplan &P, G &&g)
         case 2: return g(RtFFT<2>{});
         case 3: return g(RtFFT<3>{});
         case 4: return g(RtFFT<4>{});
+        case 5: return g(RtFFT<5>{});
         case 6: return g(RtFFT<6>{});
+        case 7: return g(RtFFT<7>{});
         case 8: return g(RtFFT<8>{});
         case 12: return g(RtFFT<12>{});
         case 24: return g(RtFFT<24>{});
@@ -1101,10 +1278,12 @@ int dispatch_plan(const pll_fftplan &P, G &&g)
 
 extern "C" int pll_fft_supported_E(int E)
 {
-    return E == 2 || E == 3 || E == 4 || E == 6 || E == 8 || E == 12 || E == 24;
+    return E == 2 || E == 3 || E == 4 || E == 5 || E == 6 || E == 7 || E == 8 || E == 12 ||
+           E == 24;
 }
 
 extern "C" int pll_fft_max_threads(int E) { return E >= 12 ? 256 : 512; }
+extern "C" int pll_fft_max_threads32(int) { return 512; }
 
 extern "C" int pll_fft_ct_plan(int L, int prefer_E, pll_fftplan *p)
 {
@@ -1140,6 +1319,15 @@ extern "C" int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, h
     });
 }
 
+extern "C" int pll_launch_xfwd_rows(const pll_ppe_dev *d, const double *p, void *W,
+                                    long long row0, long long row1, hipStream_t s)
+{
+    const SrcHalo64 src{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s, row0, row1);
+    });
+}
+
 extern "C" int pll_launch_xfwd_div(const pll_ppe_dev *d, const pll_div_src *q, void *W,
                                    hipStream_t s)
 {
@@ -1161,6 +1349,15 @@ extern "C" int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, h
     const DstHalo64 dst{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xb<decltype(f), double2>(d, d->tw_x, W, dst, s);
+    });
+}
+
+extern "C" int pll_launch_xbwd_rows(const pll_ppe_dev *d, const void *W, double *p,
+                                    long long row0, long long row1, hipStream_t s)
+{
+    const DstHalo64 dst{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    return dispatch_plan(d->px, [&](auto f) {
+        return launch_xb<decltype(f), double2>(d, d->tw_x, W, dst, s, nullptr, row0, row1);
     });
 }
 

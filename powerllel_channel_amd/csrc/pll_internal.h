@@ -15,7 +15,8 @@ extern "C" {
 #define PLL_MAX_STAGES 12
 
 /* One complex FFT of length L done by T threads holding E points each
- * (L = T*E), in nst radix stages rad[0..nst-1] (each radix divides E). */
+ * (L = T*E), in nst radix stages rad[0..nst-1]; a radix that does not divide
+ * E runs its stage through LDS. */
 typedef struct {
     int L, E, T, nst;
     int rad[PLL_MAX_STAGES];
@@ -39,6 +40,8 @@ typedef struct {
     pll_fftplan px, py;
     int x_rows_per_wg;         /* rows per workgroup, x kernels     */
     int y_cols_per_wg;         /* columns per workgroup, y kernels  */
+    int y_cols32_per_wg;       /* the same for the FP32 y kernels   */
+    int x_xcd;                 /* x kernels numbered per XCD: 0 none, 1 FP32, 2 all */
     const void *tw_x;          /* double2[nx]: exp(-2 pi i t / nx)  */
     const void *tw_y;          /* double2[ny]: exp(-2 pi i t / ny)  */
     const void *tw_x32;        /* float2[nx], same table in FP32 (refined form) */
@@ -122,17 +125,32 @@ int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s);
+/* x transforms of the rows [row0, row1) only (row = k ny + j): the
+ * transfer pipeline of host-pointer solves runs them per chunk of planes. */
+int pll_launch_xfwd_rows(const pll_ppe_dev *d, const double *p, void *W, long long row0,
+                         long long row1, hipStream_t s);
+int pll_launch_xbwd_rows(const pll_ppe_dev *d, const void *W, double *p, long long row0,
+                         long long row1, hipStream_t s);
+/* host memcpy split over a pool of threads (host-pointer solves) */
+typedef struct pll_copy_pool pll_copy_pool;
+int pll_copy_pool_create(pll_copy_pool **out, int nthreads);
+void pll_copy_pool_memcpy(pll_copy_pool *p, void *dst, const void *src, size_t n);
+void pll_copy_pool_destroy(pll_copy_pool *p);
 /* pll_launch_xbwd that also writes p's boundary fill (updateBoundP). */
 int pll_launch_xbwd_fill(const pll_ppe_dev *d, const void *W, double *p, hipStream_t s);
 /* 1 if the partitioned z solve is instantiated for M rows per lane. */
 int pll_trid_part_supported(int M);
 /* 1 if a kernel instantiation exists for E (elements per thread). */
 int pll_fft_supported_E(int E);
-/* largest workgroup the E instantiation is compiled for */
+/* largest workgroup the E instantiation is compiled for (FP64 kernels;
+ * the FP32 ones take pll_fft_max_threads32) */
 int pll_fft_max_threads(int E);
+int pll_fft_max_threads32(int E);
 /* compile-time plan for length L, if one exists (returns 1 and fills p);
  * among several, the one with E == prefer_E, else the first listed */
 int pll_fft_ct_plan(int L, int prefer_E, pll_fftplan *p);
+/* the plan the solver uses for length L (host only): E, stage count, radices */
+int pll_fft_plan_info(int L, int *E, int *nst, int *rad);
 
 /* Grid constants for the step kernels (device pointers for z arrays). */
 typedef struct {

@@ -42,9 +42,46 @@ int pll_set_error(int code, const char *fmt, ...)
     } while (0)
 
 /* ------------------------------------------------------------------ plans */
-/* Factor L = 2^a 3^b into radix stages from {8,6,4,3,2}, each dividing E,
- * taking the first E in {12,8,24,6,4,3,2} that divides L (E = 24 needs more
- * than 256 VGPRs per lane, so it is the last resort of the large ones). */
+/* Factor L = 2^a 3^b 5^c 7^d 11^e 13^f into radix stages from
+ * {8,6,4,3,2,5,7,11,13}.  E (points per thread) is the first of
+ * {12,8,24,6,4,3,5,7,2} that divides L, keeps L/E threads per row within the
+ * kernel's limit and divides every radix of the plan, so that all stages run
+ * from registers (E = 24 needs more than 256 VGPRs per lane, so it is the
+ * last resort of the large ones).  Lengths that no such E fits (mixed
+ * prime factors, such as 2^a 5 or 2^a 3 7) take the first E that divides L;
+ * their stages whose radix does not divide E run through LDS. */
+static int plan_for_E(int L, int E, const int cnt[4], const int cnt_big[2], int reg_only,
+                      pll_fftplan *p)
+{
+    int a = cnt[0], b = cnt[1], n = 0, prod = 1;
+    int rad[32];
+    while (a >= 3 && E % 8 == 0) { rad[n++] = 8; a -= 3; }
+    while (a >= 2 && E % 4 == 0) { rad[n++] = 4; a -= 2; }
+    if (a == 1 && b >= 1 && E % 6 == 0) { rad[n++] = 6; a = 0; b--; }
+    while (a >= 1 && E % 2 == 0) { rad[n++] = 2; a -= 1; }
+    while (b >= 1 && E % 3 == 0) { rad[n++] = 3; b -= 1; }
+    /* the rest through LDS */
+    while (a >= 3) { rad[n++] = 8; a -= 3; }
+    while (a >= 1) { rad[n++] = a >= 2 ? 4 : 2; a -= a >= 2 ? 2 : 1; }
+    while (b >= 1) { rad[n++] = 3; b -= 1; }
+    for (int i = 0; i < cnt[2]; i++) rad[n++] = 5;
+    for (int i = 0; i < cnt[3]; i++) rad[n++] = 7;
+    for (int i = 0; i < cnt_big[0]; i++) rad[n++] = 11;
+    for (int i = 0; i < cnt_big[1]; i++) rad[n++] = 13;
+    if (n > PLL_MAX_STAGES) return 0;
+    for (int s = 0; s < n; s++) {
+        prod *= rad[s];
+        if (reg_only && E % rad[s]) return 0;
+    }
+    if (prod != L) return 0;
+    p->L = L;
+    p->E = E;
+    p->T = L / E;
+    p->nst = n;
+    memcpy(p->rad, rad, sizeof(int) * (size_t)n);
+    return 1;
+}
+
 static int make_plan(int L, pll_fftplan *p)
 {
     memset(p, 0, sizeof *p);
@@ -52,33 +89,23 @@ static int make_plan(int L, pll_fftplan *p)
     const char *pe = getenv("PLL_FFT_E");
     if (pll_fft_ct_plan(L, pe ? atoi(pe) : 0, p)) return PLL_OK;
     memset(p, 0, sizeof *p);
-    int a2 = 0, b3 = 0, m = L;
     if (L < 2) return pll_set_error(PLL_EUNSUP, "FFT length %d < 2", L);
-    while (m % 2 == 0) { m /= 2; a2++; }
-    while (m % 3 == 0) { m /= 3; b3++; }
+    static const int primes[6] = {2, 3, 5, 7, 11, 13};
+    int c6[6] = {0}, m = L;
+    for (int i = 0; i < 6; i++)
+        while (m % primes[i] == 0) { m /= primes[i]; c6[i]++; }
     if (m != 1)
-        return pll_set_error(PLL_EUNSUP, "FFT length %d has a prime factor > 3", L);
-    static const int Es[] = {12, 8, 24, 6, 4, 3, 2};
-    for (size_t ie = 0; ie < sizeof Es / sizeof Es[0]; ie++) {
-        int E = Es[ie];
-        if (L % E != 0 || !pll_fft_supported_E(E)) continue;
-        int a = a2, b = b3, n = 0, prod = 1, ok = 1;
-        int rad[PLL_MAX_STAGES];
-        while (a >= 3 && E % 8 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 8; a -= 3; }
-        while (a >= 2 && E % 4 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 4; a -= 2; }
-        if (a == 1 && b >= 1 && E % 6 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 6; a = 0; b--; }
-        while (a >= 1 && E % 2 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 2; a -= 1; }
-        while (b >= 1 && E % 3 == 0 && n < PLL_MAX_STAGES) { rad[n++] = 3; b -= 1; }
-        if (a != 0 || b != 0) ok = 0;
-        for (int s = 0; s < n; s++) prod *= rad[s];
-        if (!ok || prod != L) continue;
-        p->L = L;
-        p->E = E;
-        p->T = L / E;
-        p->nst = n;
-        memcpy(p->rad, rad, sizeof(int) * (size_t)n);
-        return PLL_OK;
-    }
+        return pll_set_error(PLL_EUNSUP,
+                             "FFT length %d has a prime factor above 13 (radices 2, 3, 5, 7, "
+                             "11 and 13 are supported)", L);
+    const int cnt[4] = {c6[0], c6[1], c6[2], c6[3]}, big[2] = {c6[4], c6[5]};
+    static const int Es[] = {12, 8, 24, 6, 4, 3, 5, 7, 2};
+    for (int reg_only = 1; reg_only >= 0; reg_only--)
+        for (size_t ie = 0; ie < sizeof Es / sizeof Es[0]; ie++) {
+            const int E = Es[ie];
+            if (L % E != 0 || !pll_fft_supported_E(E) || L / E > pll_fft_max_threads(E)) continue;
+            if (plan_for_E(L, E, cnt, big, reg_only, p)) return PLL_OK;
+        }
     return pll_set_error(PLL_EUNSUP, "no FFT plan for length %d", L);
 }
 
@@ -144,6 +171,7 @@ static double rowsum3(double a, double b, double c)
 
 /* ------------------------------------------------------------------ handle */
 #define PLL_TIMING_RING 128
+#define PLL_HOST_SLOTS 3
 struct pll_ppe {
     int nx, ny, nz, L;
     pll_ppe_dev d;
@@ -159,6 +187,14 @@ struct pll_ppe {
     double last_est;                  /* the last solve                              */
     double *stage;   /* haloed staging buffer for host-pointer calls */
     size_t stage_bytes;
+    /* host-pointer calls: pinned staging ring, copy/compute streams, events,
+     * the host copy threads (all made by the first host-pointer call) */
+    void *pin[PLL_HOST_SLOTS];
+    size_t slot_planes;
+    hipStream_t hs_copy, hs_comp;
+    hipEvent_t ev_up[PLL_HOST_SLOTS], ev_x[PLL_HOST_SLOTS], ev_dn[PLL_HOST_SLOTS];
+    int host_ready;
+    pll_copy_pool *pool;
     size_t dev_bytes;
     int timing;
     int nrec;                             /* solves recorded since timing on */
@@ -187,6 +223,20 @@ int pll_ppe_destroy(pll_ppe *h)
     if (h->timing)
         for (int r = 0; r < PLL_TIMING_RING; r++)
             for (int i = 0; i < 6; i++) hipEventDestroy(h->ev[r][i]);
+    if (h->host_ready) {
+        hipStreamSynchronize(h->hs_copy);
+        hipStreamSynchronize(h->hs_comp);
+        for (int i = 0; i < PLL_HOST_SLOTS; i++) {
+            hipEventDestroy(h->ev_up[i]);
+            hipEventDestroy(h->ev_x[i]);
+            hipEventDestroy(h->ev_dn[i]);
+        }
+        hipStreamDestroy(h->hs_copy);
+        hipStreamDestroy(h->hs_comp);
+    }
+    for (int i = 0; i < PLL_HOST_SLOTS; i++)
+        if (h->pin[i]) hipHostFree(h->pin[i]);
+    pll_copy_pool_destroy(h->pool);
     free(h);
     return PLL_OK;
 }
@@ -216,6 +266,24 @@ static int choose_launch(pll_ppe *h)
     if (cols * py->T > maxt || (size_t)cols * (py->L + 1) * 16 > 81920)
         return pll_set_error(PLL_EUNSUP, "ny=%d too large for the y transform kernel", h->ny);
     h->d.y_cols_per_wg = cols;
+    /* FP32 y kernels: 16 complex columns make 128-byte row segments (8 would
+     * fetch each line twice); their workgroups may hold 512 threads */
+    int cols32 = 16;
+    {
+        const char *pc = getenv("PLL_Y_COLS32"); /* tuning override */
+        if (pc && atoi(pc) > 0) cols32 = atoi(pc);
+    }
+    if (cols32 > h->L) cols32 = h->L;
+    maxt = pll_fft_max_threads32(py->E);
+    while (cols32 > 1 && (cols32 * py->T > maxt || (size_t)cols32 * (py->L + 1) * 8 > 81920)) cols32--;
+    h->d.y_cols32_per_wg = cols32 < 1 ? 1 : cols32;
+    /* x kernels of the refined form: workgroups numbered per XCD, so that the
+     * rows of neighbouring planes (read again by the residual) share an L2
+     * (refined k = 2 at 768x384x256: 2.97 -> 2.89 ms with the 16-column FP32
+     * y tiles); the FP64 x kernels run 3-5 us slower that way.
+     * PLL_X_XCD: 0 none, 1 FP32 kernels (default), 2 all */
+    const char *px_ = getenv("PLL_X_XCD");
+    h->d.x_xcd = px_ ? atoi(px_) : 1;
     return PLL_OK;
 }
 
@@ -855,6 +923,107 @@ int execute_poisson_solver_dev(double *d_var, void *stream)
     return pll_ppe_solve(g_ppe, d_var, stream);
 }
 
+/* Host-pointer solves (the reference's own call pattern, src/main.f90:177):
+ * the interior planes of the haloed array (k = 1..nz, each a contiguous
+ * (nx+2)(ny+2) block) go to the device in chunks through a ring of pinned
+ * buffers: host threads copy chunk c into a pinned slot while the DMA engine
+ * moves chunk c-1, and the x forward transform of a chunk starts as soon as
+ * its planes have arrived.  After the y transforms and the z solve, the x
+ * backward transform runs per chunk and each chunk goes back through the
+ * ring the same way.  Ghost planes are neither sent nor written; the halo
+ * cells inside the interior planes travel both ways unchanged.  With the
+ * refined form switched on, the whole array is sent, solved and returned
+ * through the same ring without the per-chunk transforms.
+ * PLL_HOST_PIPE=0 selects one plain hipMemcpy each way instead. */
+static int host_setup(pll_ppe *h)
+{
+    if (h->host_ready) return PLL_OK;
+    const size_t plane = sizeof(double) * (size_t)(h->nx + 2) * (h->ny + 2);
+    /* slots of at most 20 MiB (whole planes): small chunks start the first
+     * transform and the last download sooner (768x384x256: 8 planes 24.0 ms
+     * per solve, 26 planes 25.3 ms) */
+    size_t cp = ((size_t)20 << 20) / plane;
+    if (cp < 1) cp = 1;
+    if (cp > (size_t)h->nz) cp = (size_t)h->nz;
+    const char *pc = getenv("PLL_HOST_CHUNK"); /* tuning: planes per chunk */
+    if (pc && atoi(pc) > 0) cp = (size_t)atoi(pc) < (size_t)h->nz ? (size_t)atoi(pc) : (size_t)h->nz;
+    for (int i = 0; i < PLL_HOST_SLOTS; i++)
+        if (!h->pin[i]) HIPCHK(hipHostMalloc(&h->pin[i], cp * plane, hipHostMallocDefault));
+    h->slot_planes = cp;
+    HIPCHK(hipStreamCreateWithFlags(&h->hs_copy, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->hs_comp, hipStreamNonBlocking));
+    for (int i = 0; i < PLL_HOST_SLOTS; i++) {
+        HIPCHK(hipEventCreateWithFlags(&h->ev_up[i], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&h->ev_x[i], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&h->ev_dn[i], hipEventDisableTiming));
+    }
+    int nt = 8;
+    const char *pt = getenv("PLL_HOST_THREADS"); /* host copy threads */
+    if (pt && atoi(pt) > 0) nt = atoi(pt);
+    if (!h->pool && pll_copy_pool_create(&h->pool, nt) != 0)
+        return pll_set_error(PLL_ENOMEM, "host copy threads could not be created");
+    h->host_ready = 1;
+    return PLL_OK;
+}
+
+static int host_solve(pll_ppe *h, double *var)
+{
+    int rc;
+    if ((rc = host_setup(h))) return rc;
+    const long long off1 = (long long)(h->nx + 2) * (h->ny + 2);
+    const size_t plane = sizeof(double) * (size_t)off1;
+    const int nz = h->nz, ny = h->ny, cp = (int)h->slot_planes;
+    const int nch = (nz + cp - 1) / cp;
+    const int fp64 = h->sweeps < 0 && !(h->tol > 0.0);
+    hipStream_t cs = h->hs_copy, ks = h->hs_comp;
+    /* upload (and x forward per chunk) */
+    for (int c = 0; c < nch; c++) {
+        const int k0 = c * cp, k1 = k0 + cp < nz ? k0 + cp : nz, sl = c % PLL_HOST_SLOTS;
+        const size_t bytes = plane * (size_t)(k1 - k0);
+        if (c >= PLL_HOST_SLOTS) HIPCHK(hipEventSynchronize(h->ev_up[sl]));
+        pll_copy_pool_memcpy(h->pool, h->pin[sl], var + (k0 + 1) * off1, bytes);
+        HIPCHK(hipMemcpyAsync(h->stage + (k0 + 1) * off1, h->pin[sl], bytes,
+                              hipMemcpyHostToDevice, cs));
+        HIPCHK(hipEventRecord(h->ev_up[sl], cs));
+        HIPCHK(hipStreamWaitEvent(ks, h->ev_up[sl], 0));
+        if (fp64 && (rc = launch_rc(pll_launch_xfwd_rows(&h->d, h->stage, h->W, (long long)k0 * ny,
+                                                         (long long)k1 * ny, ks),
+                                    "x forward")))
+            return rc;
+    }
+    if (fp64) {
+        if ((rc = launch_rc(pll_launch_yfwd(&h->d, h->W, ks), "y forward")) ||
+            (rc = launch_rc(pll_launch_trid(&h->d, h->W, ks), "z solve")) ||
+            (rc = launch_rc(pll_launch_ybwd(&h->d, h->W, ks), "y backward")))
+            return rc;
+    } else if ((rc = pll_ppe_solve(h, h->stage, ks))) {
+        return rc;
+    }
+    /* x backward per chunk (FP64) and download; a slot is reused only after
+     * the host has copied its previous chunk out */
+    for (int c = 0; c < nch + PLL_HOST_SLOTS; c++) {
+        const int cd = c - PLL_HOST_SLOTS; /* chunk to drain before slot reuse */
+        if (cd >= 0) {
+            const int k0 = cd * cp, k1 = k0 + cp < nz ? k0 + cp : nz, sl = cd % PLL_HOST_SLOTS;
+            HIPCHK(hipEventSynchronize(h->ev_dn[sl]));
+            pll_copy_pool_memcpy(h->pool, var + (k0 + 1) * off1, h->pin[sl],
+                                 plane * (size_t)(k1 - k0));
+        }
+        if (c >= nch) continue;
+        const int k0 = c * cp, k1 = k0 + cp < nz ? k0 + cp : nz, sl = c % PLL_HOST_SLOTS;
+        if (fp64 && (rc = launch_rc(pll_launch_xbwd_rows(&h->d, h->W, h->stage, (long long)k0 * ny,
+                                                         (long long)k1 * ny, ks),
+                                    "x backward")))
+            return rc;
+        HIPCHK(hipEventRecord(h->ev_x[sl], ks));
+        HIPCHK(hipStreamWaitEvent(cs, h->ev_x[sl], 0));
+        HIPCHK(hipMemcpyAsync(h->pin[sl], h->stage + (k0 + 1) * off1, plane * (size_t)(k1 - k0),
+                              hipMemcpyDeviceToHost, cs));
+        HIPCHK(hipEventRecord(h->ev_dn[sl], cs));
+    }
+    return PLL_OK;
+}
+
 int execute_poisson_solver_status(double *var)
 {
     pll_ppe *h = g_ppe;
@@ -871,6 +1040,8 @@ int execute_poisson_solver_status(double *var)
         if ((rc = dmalloc(h, (void **)&h->stage, bytes))) return rc;
         h->stage_bytes = bytes;
     }
+    const char *pp = getenv("PLL_HOST_PIPE");
+    if (!(pp && atoi(pp) == 0)) return host_solve(h, var);
     HIPCHK(hipMemcpy(h->stage, var, bytes, hipMemcpyHostToDevice));
     if ((rc = pll_ppe_solve(h, h->stage, NULL))) return rc;
     HIPCHK(hipMemcpy(var, h->stage, bytes, hipMemcpyDeviceToHost));
@@ -914,3 +1085,16 @@ void execute_poisson_solver(double *var)
 }
 
 void free_poisson_solver(void) { abort_on(free_poisson_solver_status(), "free_poisson_solver"); }
+
+/* Host-only query of the FFT plan chosen for length L (no device needed):
+ * points per thread, stage count and radices (rad: PLL_MAX_STAGES ints). */
+int pll_fft_plan_info(int L, int *E, int *nst, int *rad)
+{
+    pll_fftplan p;
+    const int rc = make_plan(L, &p);
+    if (rc) return rc;
+    *E = p.E;
+    *nst = p.nst;
+    for (int s = 0; s < p.nst; s++) rad[s] = p.rad[s];
+    return PLL_OK;
+}

@@ -16,7 +16,7 @@ mathematics, never from the oracle or the GPU code:
                        assembled as a dense matrix and solved by least
                        squares (minimum norm for the singular N,N case).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fft] [ppe]
 """
 import math
 import os
@@ -24,7 +24,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-NS = [2, 4, 6, 8, 10, 12, 16, 18, 24, 32, 36, 48, 64, 96, 128, 192, 384, 768, 5, 9, 15]
+NS = [2, 4, 6, 8, 10, 12, 16, 18, 24, 32, 36, 48, 64, 96, 128, 192, 384, 768, 5, 9, 15,
+      # lengths with the prime factors 5, 7, 11 and 13 (x rows of nx = 2L, y columns)
+      7, 11, 13, 14, 20, 22, 26, 35, 40, 49, 50, 70, 80, 98, 110, 120, 140, 160, 320]
 
 
 def halfcomplex(x):
@@ -92,7 +94,7 @@ def ppe_case(name, nx, ny, nz, sr, bcz, seed):
              dzf=dzf, bcz=bcz, rhs=f.reshape(nz, ny, nx), p=p.reshape(nz, ny, nx))
 
 
-def main():
+def fft_fixture():
     rng = np.random.default_rng(2024)
     data = {}
     for n in NS:
@@ -100,11 +102,20 @@ def main():
         data[f"x{n}"] = x
         data[f"hc{n}"] = halfcomplex(x)
     np.savez(os.path.join(HERE, "fft_halfcomplex.npz"), **data)
+
+
+def main(which=("fft", "ppe")):
+    if "fft" in which:
+        fft_fixture()
+    if "ppe" not in which:
+        return
     ppe_case("8x6x5_NN_sr3", 8, 6, 5, 3.0, "NN", 1)
     ppe_case("6x4x6_NN_uniform", 6, 4, 6, 0.0, "NN", 2)
     ppe_case("4x6x7_DD_sr4", 4, 6, 7, 4.0, "DD", 3)
     ppe_case("8x4x6_ND_sr2", 8, 4, 6, 2.0, "ND", 4)
+    ppe_case("10x14x4_NN_sr3", 10, 14, 4, 3.0, "NN", 5)  # x length 5, y length 14 = 2*7
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    main(sys.argv[1:] or ("fft", "ppe"))

@@ -53,12 +53,48 @@ def _create(lib, nx, ny, nz, bcx=b"PP", bcy=b"PP", bcz=b"NN", dzf=None):
     ((7, 8, 8), -1),              # odd nx
     ((8, 7, 8), -1),              # odd ny
     ((8, 8, 1), -1),              # nz < 2
-    ((22, 8, 8), -5),             # nx/2 = 11: prime factor above 7
+    ((34, 8, 8), -5),             # nx/2 = 17: prime factor above 13
+    ((8, 38, 8), -5),             # ny = 2 * 19
+    ((242, 8, 8), -5),            # nx/2 = 121 = 11^2: no points-per-thread count divides it
 ])
 def test_create_rejects_bad_sizes(lib, args, code):
     rc, h = _create(lib, *args)
     assert rc == code and not h.value
     assert lib.pll_last_error()
+
+
+def _plan(lib, L):
+    E, n, rad = C.c_int(), C.c_int(), (C.c_int * 12)()
+    rc = lib.pll_fft_plan_info(L, C.byref(E), C.byref(n), rad)
+    return rc, E.value, list(rad[: n.value])
+
+
+def test_fft_plans_cover_lengths(lib):
+    """Every transform length whose prime factors are 2, 3, 5, 7, 11 and 13
+    gets a plan (up to 4096) whose radices multiply to the length, when a
+    points-per-thread count E of the instantiated set divides it with at most
+    the kernel's thread limit per row (odd lengths built only from 11 and 13,
+    or 3^7, have none); lengths with a larger prime factor are refused."""
+    for L in range(2, 4097):
+        m = L
+        for q in (2, 3, 5, 7, 11, 13):
+            while m % q == 0:
+                m //= q
+        rc, E, rad = _plan(lib, L)
+        if m != 1:
+            assert rc == -5, L
+            continue
+        if not any(L % E == 0 and L // E <= (256 if E >= 12 else 512)
+                   for E in (2, 3, 4, 5, 6, 7, 8, 12, 24)):
+            assert rc == -5, L
+            continue
+        assert rc == 0, L
+        assert L % E == 0 and int(np.prod(rad)) == L, (L, E, rad)
+        assert all(x in (2, 3, 4, 5, 6, 7, 8, 11, 13) for x in rad), (L, rad)
+    # lengths of the production grids keep all-register plans (E divisible by every radix)
+    for L in (32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024):
+        rc, E, rad = _plan(lib, L)
+        assert rc == 0 and all(E % x == 0 for x in rad), (L, E, rad)
 
 
 @pytest.mark.parametrize("bcx,bcy,bcz", [(b"NN", b"PP", b"NN"), (b"PP", b"DD", b"NN"),

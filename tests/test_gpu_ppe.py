@@ -32,6 +32,17 @@ CASES = [
     # transform lengths of the 1536x1024x384 grid: x 768 (nx/2), y 1024; y 768
     (1536, 1024, 8, 4.4),
     (64, 768, 16, 4.4),
+    # lengths with the prime factors 5, 7, 11, 13 (FFTW plans any length):
+    # x 80 = 2^4 5, y 120 = 2^3 3 5 (radix-5 stages through LDS)
+    (160, 120, 64, 3.0),
+    (80, 40, 40, 4.0),
+    # x 70 = 2 5 7, y 70; x 50 = 2 5^2, y 50; x 49 = 7^2 (odd), y 42 = 2 3 7
+    (140, 70, 32, 3.0),
+    (100, 50, 24, 4.0),
+    (98, 42, 16, 3.0),
+    # x 55 = 5 11, y 26 = 2 13; x 5 (one thread per row), y 14 = 2 7
+    (110, 26, 16, 3.0),
+    (10, 14, 8, 3.0),
 ]
 
 
@@ -174,6 +185,50 @@ def test_dropin_host_and_device(orc, cuda):
         ppe.free_poisson_solver()
 
 
+@pytest.mark.parametrize("chunk", ["1", "3", "7", "0"])
+@pytest.mark.parametrize("refine", [-1, 2])
+def test_dropin_host_pipeline(cuda, chunk, refine, monkeypatch):
+    """Host-pointer calls move the interior planes through a ring of three
+    pinned slots in chunks of planes, with the x transforms run per chunk;
+    chunks of 1, 3 and 7 planes of nz = 20 (a ragged last chunk, more chunks
+    than slots) and the default give the device path's answer bit for bit,
+    halos and ghost planes untouched, on three calls in a row (slot reuse),
+    with the FP64 solve and with the refined form; PLL_HOST_PIPE=0 (one copy
+    each way) gives the same."""
+    import torch
+    from powerllel_channel_amd import ppe
+    nx, ny, nz, sr = 48, 36, 20, 3.0
+    if chunk != "0":
+        monkeypatch.setenv("PLL_HOST_CHUNK", chunk)
+    g0, _ = _case(nx, ny, nz, sr)
+    results = {}
+    for pipe in ("1", "0"):
+        monkeypatch.setenv("PLL_HOST_PIPE", pipe)
+        ppe.init_poisson_solver(nx, ny, nz, g0.dx, g0.dy, g0.dzf)
+        try:
+            if refine >= 0:
+                ppe.set_poisson_solver_refinement(refine)
+            outs = []
+            for call in range(3):
+                g, rhs = _case(nx, ny, nz, sr, seed=30 + call)
+                host = rhs.copy()
+                ppe.execute_poisson_solver(host)
+                dev = torch.from_numpy(rhs.copy()).to(cuda)
+                ppe.execute_poisson_solver(dev)
+                d = dev.cpu().numpy()
+                assert np.array_equal(host, d), (pipe, call, float(np.abs(host - d).max()))
+                # halos and ghost planes as given
+                inner = np.zeros(rhs.shape, bool)
+                inner[1:-1, 1:-1, 1:-1] = True
+                assert np.array_equal(host[~inner], rhs[~inner])
+                outs.append(host)
+            results[pipe] = outs
+        finally:
+            ppe.free_poisson_solver()
+    for a, b in zip(results["1"], results["0"]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("nx,ny,nz,sr", [(768, 384, 256, 4.0), (256, 192, 128, 3.0),
                                          (96, 64, 384, 4.4)])
 def test_partitioned_vs_streaming_zsolve(cuda, nx, ny, nz, sr, monkeypatch):
@@ -212,7 +267,8 @@ REFINED_BAR = {0: 1e-5, 1: 1e-9, 2: 1e-11, 3: 5e-12}
 
 
 @pytest.mark.parametrize("nx,ny,nz,sr", [(768, 384, 256, 4.0), (192, 192, 128, 3.0),
-                                         (48, 36, 24, 3.0), (1536, 1024, 8, 4.4)])
+                                         (48, 36, 24, 3.0), (1536, 1024, 8, 4.4),
+                                         (160, 70, 48, 3.0)])
 def test_refined_error_per_sweep(cuda, nx, ny, nz, sr):
     import torch
     from powerllel_channel_amd import PoissonSolver

@@ -189,6 +189,52 @@ def test_steps_full_size_vs_oracle(orc, cuda):
     ch.close()
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(160, 70, 48), (110, 26, 16)])
+def test_steps_radix57_vs_oracle(orc, cuda, nx, ny, nz, monkeypatch):
+    """Grids whose transform lengths have the prime factors 5, 7, 11 and 13
+    (x 80 = 2^4 5, y 70 = 2 5 7; x 55 = 5 11, y 26 = 2 13), stepped through
+    pll_channel_run with the projection carried into the next step's RK1 (the
+    fused right-hand side read inside the x transform), against the oracle.
+    After one step the right-hand sides agree bit for bit (the stencils are
+    bit-exact), so the pressure is held to the solve's 1e-12.  After two more,
+    velocity is held to 1e-12 and pressure to 1e-11: the velocities then
+    differ by roundoff (about 2e-15), which the pressure right-hand side
+    divides by dx dt."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, mean_removed, poiseuille_fields
+    monkeypatch.setenv("PLL_RUN_CHAIN", "1")
+    nu, dt = 1 / 5600, 1e-3
+    print(f"[steps radix 5/7/11/13] {nx}x{ny}x{nz}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, 3.0)
+    G = orc.OracleGrid(nx, ny, nz, LX, LY, LZ, g.dzf, nu, dt)
+    u, v, w = poiseuille_fields(g, 23)
+    orc.bc_vel(G, u, v, w, 0.0)
+    u1, v1, w1, p = [np.zeros(g.shape) for _ in range(4)]
+    d = _dev(torch, cuda, u, v, w, u1, v1, w1, p)
+    ch = Channel(nx, ny, nz, LX, LY, LZ, g.dzf, nu, dt)
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    ppe = orc.OraclePPE(nx, ny, nz, g.dx, g.dy, g.dzf, pin_gauge=True)
+    for nsteps, ptol in ((1, 1e-12), (2, 1e-11)):
+        ch.run(s, nsteps, *d)
+        for _ in range(nsteps):
+            orc.step(G, ppe, u, v, w, u1, v1, w1, p)
+        torch.cuda.synchronize()
+        div, _ = ch.max_div(d[0], d[1], d[2])
+        for name, t, ref in zip("uvw", d[:3], (u, v, w)):
+            rel = float(np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max())
+            print(f"  +{nsteps} step(s): {name} rel {rel:.2e}", flush=True)
+            assert rel <= 1e-12, (name, rel)
+        gp = d[6].cpu().numpy()
+        ep = float(np.abs(mean_removed(g, gp) - mean_removed(g, p)).max()
+                   / np.abs(mean_removed(g, p)).max())
+        print(f"  +{nsteps} step(s): p rel (mean-removed) {ep:.2e}, max|div u| {div:.2e}")
+        assert ep <= ptol and div < 1e-12
+    s.close()
+    ch.close()
+
+
 def _step_vs_parts(orc, cuda, nx, ny, nz, is_forced):
     """One fused step and the same step as separate calls, from one start."""
     import torch
