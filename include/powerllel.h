@@ -210,6 +210,25 @@ int pll_channel_run(pll_channel *c, pll_ppe *ppe, int nsteps, double *u,
                     double *v, double *w, double *u1, double *v1, double *w1,
                     double *p, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, hosts that forbid
+ * hipMalloc): calcMaxDiv, src/mod_utils.f90:257-318, with the results of
+ * pll_channel_max_div, but without a channel handle and without any device
+ * allocation.  u, v, w are haloed device fields, dzf_global a device array
+ * of nz+2 cell heights, dx = lx/nx and dy = ly/ny the cell widths.  Its work
+ * space (1/dzf and one partial maximum per (j,k) row) is a static device
+ * array of PLL_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends:
+ * PLL_EINVAL unless ny*nz <= PLL_NOALLOC_MAX_ROWS and nz+2 <=
+ * PLL_NOALLOC_MAX_NZ2.  Calls on one GPU share that array, so calls that
+ * overlap on one GPU race.  Synchronises `stream`. */
+#define PLL_NOALLOC_MAX_ROWS 524288
+#define PLL_NOALLOC_MAX_NZ2 8192
+#define PLL_NOALLOC_SCRATCH_BYTES (8 * PLL_NOALLOC_MAX_NZ2 + 16 * PLL_NOALLOC_MAX_ROWS + 256)
+int pll_max_div_noalloc(const double *u, const double *v, const double *w,
+                        int nx, int ny, int nz, double dx, double dy,
+                        const double *dzf_global, double *value, int ijk[3],
+                        void *stream);
+
 /* ------------------------------------------------------------------------
  * Adjoint of the projection step (no counterpart in the reference).
  * Every array is a haloed device field like u.

@@ -66,6 +66,8 @@ _SIGS = {
     "pll_channel_bc_p": (_ip, [_vp, _dp, _vp]),
     "pll_channel_mean_h": (_ip, [_vp, _dp, C.POINTER(_d), _vp]),
     "pll_channel_max_div": (_ip, [_vp, _dp, _dp, _dp, C.POINTER(_d), C.POINTER(_ip), _vp]),
+    "pll_max_div_noalloc": (_ip, [_dp, _dp, _dp, _ip, _ip, _ip, _d, _d, _dp, C.POINTER(_d),
+                                  C.POINTER(_ip), _vp]),
     "pll_channel_step": (_ip, [_vp, _vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "pll_channel_run": (_ip, [_vp, _vp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "pll_channel_max_cfl": (_ip, [_vp, _dp, _dp, _dp, C.POINTER(_d), C.POINTER(_ip), _vp]),

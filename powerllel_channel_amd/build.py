@@ -32,6 +32,8 @@ HIP_SOURCES = {
     "adjoint_kernels.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
     # restates the reference's Fortran arithmetic: no fused multiply-add
     "step_kernels.hip": ["-ffp-contract=off"],
+    # allocation-free calcMaxDiv: its static device scratch array and launcher
+    "noalloc.hip": [],
 }
 C_SOURCES = ["ppe.c", "channel.c", "adjoint.c", "hostcopy.c"]
 
