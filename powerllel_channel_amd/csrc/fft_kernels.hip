@@ -591,21 +591,28 @@ struct SrcResid {
         const double *__restrict__ row = P + pb + k * p1 + j * p2;
         const double *__restrict__ rjp = P + pb + k * p1 + jp * p2;
         const double *__restrict__ rjm = P + pb + k * p1 + jm * p2;
-        const double c0 = row[i0], c1 = row[i1];
+        // pairs (i0, i1) as one 16-byte access each (8-byte aligned in a
+        // haloed array)
+        auto pair = [](const double *q) { return *reinterpret_cast<const d2u *>(q); };
+        const d2u cc = pair(row + i0), yp = pair(rjp + i0), ym = pair(rjm + i0);
+        const double c0 = cc.x, c1 = cc.y;
         const double xm = row[im], xp = row[ip];
-        const double yp0 = rjp[i0], yp1 = rjp[i1], ym0 = rjm[i0], ym1 = rjm[i1];
+        const double yp0 = yp.x, yp1 = yp.y, ym0 = ym.x, ym1 = ym.y;
         double zm0 = 0.0, zm1 = 0.0, zp0 = 0.0, zp1 = 0.0;
         if (k > 0) {
-            zm0 = row[i0 - p1];
-            zm1 = row[i1 - p1];
+            const d2u zm = pair(row + i0 - p1);
+            zm0 = zm.x;
+            zm1 = zm.y;
         }
         if (k + 1 < nz) {
-            zp0 = row[i0 + p1];
-            zp1 = row[i1 + p1];
+            const d2u zp = pair(row + i0 + p1);
+            zp0 = zp.x;
+            zp1 = zp.y;
         }
         const double ak = a[k], ck = c[k], dk = d[k];
         const double *__restrict__ fr = f + fb + k * f1 + j * f2;
-        const double f0 = fr[i0], fv1 = fr[i1];
+        const d2u ff = pair(fr + i0);
+        const double f0 = ff.x, fv1 = ff.y;
         const double lx0 = ((c1 - c0) + (xm - c0)) * idx2;
         const double lx1 = ((xp - c1) + (c0 - c1)) * idx2;
         const double ly0 = ((yp0 - c0) + (ym0 - c0)) * idy2;

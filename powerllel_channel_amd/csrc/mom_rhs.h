@@ -47,15 +47,24 @@ struct NbT {
 };
 using Nb = NbT<double>;
 
+// The z metric tables are never written by the kernels that read them; loads
+// through the constant address space let the scalar unit serve them (their
+// index is the same for a whole workgroup), so that they wait on the scalar
+// counter and not behind the vector loads in flight.
+__device__ __forceinline__ double zm(const double *p, int i)
+{
+    return ((const __attribute__((address_space(4))) double *)p)[i];
+}
+
 // Momentum right-hand sides (visc - conv) of the u, v and w equations at
 // interior cell (i,j,k): mod_calcVel.f90:74-138, same operation order.
 template <typename T>
 __device__ __forceinline__ void mom_rhs(const G &g, const NbT<T> &n, int k, T &fu, T &fv, T &fw)
 {
     const double dx_inv = g.dx_inv, dy_inv = g.dy_inv, re_inv = g.re_inv, u_crf = g.u_crf;
-    const double dzf_km = g.dzf[k - 1], dzf_k = g.dzf[k], dzf_kp = g.dzf[k + 1];
-    const double dzfi_k = g.dzf_inv[k], dzfi_kp = g.dzf_inv[k + 1];
-    const double dzci_k = g.dzc_inv[k], dzci_km = g.dzc_inv[k - 1];
+    const double dzf_km = zm(g.dzf, k - 1), dzf_k = zm(g.dzf, k), dzf_kp = zm(g.dzf, k + 1);
+    const double dzfi_k = zm(g.dzf_inv, k), dzfi_kp = zm(g.dzf_inv, k + 1);
+    const double dzci_k = zm(g.dzc_inv, k), dzci_km = zm(g.dzc_inv, k - 1);
     T uw, ue, us, un, ub, ut, ww, we, ws, wn, wb, wt, vw, ve, vs, vn, vb, vt;
     T dqw, dqe, dqs, dqn, dqb, dqt, conv, visc;
 

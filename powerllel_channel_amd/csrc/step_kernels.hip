@@ -12,6 +12,8 @@
 // Fields are the reference's haloed arrays f(0:nx+1,0:ny+1,0:nz+1), x fastest.
 #include <hip/hip_runtime.h>
 
+#include <stdlib.h>
+
 #include <type_traits>
 
 #include "mom_rhs.h"
@@ -99,8 +101,17 @@ __device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n
 
 // FILL: q's boundary fill has not been made; the values it would leave
 // (k_bc_vel) are read from the interior instead of the ghost cells.
-template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY, bool FILL = false>
-__global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__restrict__ qu,
+// FULL: every tile lies inside the grid (nx % 64 == 0, ny % TY == 0).  Then no
+// lane is masked: every lane loads its plane share (lanes past the halo tile
+// load and store its last element again) and the last planes of a chunk are
+// fetched again instead of skipped, so the loop has no divergent branch and
+// each wait covers only the loads it needs: the prefetch is waited for when it
+// is written to LDS (the plane's stores may stay in flight), and RK2's old
+// values of plane k+1 are loaded before the prefetch of plane k+2, so they
+// have arrived when plane k+1 is computed.
+template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY, bool FILL = false,
+          bool FULL = false>
+__global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : 1)) k_rk(G g, Tiles t, const double *__restrict__ qu,
                                                      const double *__restrict__ qv,
                                                      const double *__restrict__ qw,
                                                      double *__restrict__ ou,
@@ -119,7 +130,7 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
     const int lx = tid % RK_TX, ly = tid / RK_TX;
     const int i0 = tx * RK_TX, j0 = ty * TY;  // haloed index of the tile's halo corner
     const int i = i0 + lx + 1, j = j0 + ly + 1;
-    const bool valid = i <= g.nx && j <= g.ny;
+    const bool valid = FULL || (i <= g.nx && j <= g.ny);
     const int k0 = kc * t.kc + 1, k1 = min(k0 + t.kc - 1, g.nz);
     const long long sy = g.sy, sz = g.sz;
     const double *__restrict__ Q[3] = {qu, qv, qw};
@@ -129,10 +140,10 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
     int gii[NLD], gjj[NLD];
 #pragma unroll
     for (int r = 0; r < NLD; ++r) {
-        const int q = tid + r * NT;
+        const int q = FULL ? min(tid + r * NT, LPL - 1) : tid + r * NT;
         const int row = q / LTX, col = q - row * LTX;
         const int gi = i0 + col, gj = j0 + row;
-        gk[r] = q < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
+        gk[r] = FULL || (q < LPL && gi <= g.nx + 1 && gj <= g.ny + 1);
         go[r] = gi + sy * gj;
         gii[r] = gk[r] ? wrap(gi, g.nx) : 1;
         gjj[r] = gk[r] ? wrap(gj, g.ny) : 1;
@@ -178,20 +189,21 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
     if (CORR) {
 #pragma unroll
         for (int r = 0; r < NLP; ++r) {
-            const int q = tid + r * NT;
+            const int q = FULL ? min(tid + r * NT, LPP - 1) : tid + r * NT;
             const int row = q / PTX, col = q - row * PTX;
             const int gi = i0 + col, gj = j0 + row;
-            gpk[r] = q < LPP && gi <= g.nx + 2 && gj <= g.ny + 2;
+            gpk[r] = FULL || (q < LPP && gi <= g.nx + 2 && gj <= g.ny + 2);
             gpo[r] = gpk[r] ? wrap_p(gi, g.nx) + sy * wrap_p(gj, g.ny) : 0;
             // p of the chunk's first plane, for its first projection
-            if (q < LPP) tp[0][q] = gpk[r] ? cr.p[gpo[r] + sz * k0] : 0.0;
+            if (FULL || q < LPP) tp[0][q] = gpk[r] ? cr.p[gpo[r] + sz * k0] : 0.0;
         }
     }
-    auto fetch = [&](int kp) {
+    // interior: 1 <= kp <= nz - 1 is known (no wall rule applies to the plane)
+    auto fetch = [&](int kp, bool interior = false) {
         kpre = kp;
         if (CORR) {
-            const int ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
-            const bool above = kp >= 1 && kp <= g.nz - 1;
+            const int ks = interior ? kp : (kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp));
+            const bool above = interior || (kp >= 1 && kp <= g.nz - 1);
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
                 const long long e0 = gii[r] + sy * gjj[r] + sz * ks;
@@ -216,21 +228,24 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
                 for (int r = 0; r < NLD; ++r) pre[f][r] = gk[r] ? Q[f][go[r] + sz * kp] : 0.0;
         }
     };
-    auto put = [&](int b) {
+    // interior: the plane put lies strictly inside (1 <= kp <= nz - 1), so no
+    // wall rule of the fill applies (the FULL loop's planes but the last two)
+    auto put = [&](int b, bool interior = false) {
         if (CORR) {
             // projection (k_correct) of the plane's source values, then the
             // fill (k_bc_vel) for the wall planes and w's wall row.  tp[pcur]
             // holds p of the source plane ks; p of the plane above goes to the
             // other tile and becomes the source plane's p for the next put
-            const int kp = kpre, ks = kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp);
-            const bool wall = kp == 0 || kp == g.nz + 1;
-            const bool above = kp >= 1 && kp <= g.nz - 1;
-            const double dzi = g.dzc_inv[ks], bcu = 0.0 - g.u_crf;
+            const int kp = kpre;
+            const int ks = interior ? kp : (kp == 0 ? 1 : (kp == g.nz + 1 ? g.nz : kp));
+            const bool wall = !interior && (kp == 0 || kp == g.nz + 1);
+            const bool above = interior || (kp >= 1 && kp <= g.nz - 1);
+            const double dzi = pll::zm(g.dzc_inv, ks), bcu = 0.0 - g.u_crf;
             if (above) {
 #pragma unroll
                 for (int r = 0; r < NLP; ++r) {
-                    const int q = tid + r * NT;
-                    if (q < LPP) tp[pcur ^ 1][q] = pnext[r];
+                    const int q = FULL ? min(tid + r * NT, LPP - 1) : tid + r * NT;
+                    if (FULL || q < LPP) tp[pcur ^ 1][q] = pnext[r];
                 }
             }
             __syncthreads();
@@ -250,7 +265,7 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
                     un = 2.0 * bcu - un;
                     vn = 2.0 * 0.0 - vn;
                     wn = 0.0;
-                } else if (kp == g.nz) {
+                } else if (!interior && kp == g.nz) {
                     wn = 0.0;
                 }
                 pre[0][r] = un;
@@ -258,7 +273,7 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
                 pre[2][r] = wn;
             }
             if (above) pcur ^= 1;
-        } else if (FILL) {
+        } else if (FILL && !interior) {
             const int kp = kpre;
             if (kp == 0 || kp == g.nz + 1) {
                 const double bcu = 0.0 - g.u_crf;
@@ -277,8 +292,8 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
         for (int f = 0; f < 3; ++f)
 #pragma unroll
             for (int r = 0; r < NLD; ++r) {
-                const int q = tid + r * NT;
-                if (q < LPL) tl[b][f][q] = pre[f][r];
+                const int q = FULL ? min(tid + r * NT, LPL - 1) : tid + r * NT;
+                if (FULL || q < LPL) tl[b][f][q] = pre[f][r];
             }
     };
     fetch(k0);
@@ -322,60 +337,116 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
     __syncthreads();
     double su = 0.0, sv = 0.0;
     const int c0 = (ly + 1) * LTX + lx + 1;  // this cell in a tile
-    for (int k = k0; k <= k1; ++k, e += sz) {
-        const int b = (k - k0) & 1;
-        const bool more = k + 2 <= k1 + 1;
-        if (more) fetch(k + 2);
-        if (valid) {
-            const double *cu = tl[b][0], *cv = tl[b][1], *cw = tl[b][2];
-            const double *nu = tl[b ^ 1][0], *nv = tl[b ^ 1][1], *nw = tl[b ^ 1][2];
-            Nb n;
-            n.u000 = cu[c0]; n.up00 = cu[c0 + 1]; n.um00 = cu[c0 - 1];
-            n.u0p0 = cu[c0 + LTX]; n.u0m0 = cu[c0 - LTX]; n.ump0 = cu[c0 - 1 + LTX];
-            n.u00p = nu[c0]; n.um0p = nu[c0 - 1]; n.u00m = u00m;
-            n.v000 = cv[c0]; n.vp00 = cv[c0 + 1]; n.vm00 = cv[c0 - 1];
-            n.v0p0 = cv[c0 + LTX]; n.v0m0 = cv[c0 - LTX]; n.vpm0 = cv[c0 + 1 - LTX];
-            n.v00p = nv[c0]; n.v0mp = nv[c0 - LTX]; n.v00m = v00m;
-            n.w000 = cw[c0]; n.wp00 = cw[c0 + 1]; n.wm00 = cw[c0 - 1];
-            n.w0p0 = cw[c0 + LTX]; n.w0m0 = cw[c0 - LTX]; n.w00p = nw[c0];
-            n.w00m = w00m; n.wp0m = wp0m; n.w0pm = w0pm;
-            double fu, fv, fw;
-            mom_rhs(g, n, k, fu, fv, fw);
-            if (BODY) {
-                fu = fu + fb.u[e];
-                fv = fv + fb.v[e];
-                fw = fw + fb.w[e];
-            }
-            if (SECOND) {
-                const double un = (n.u000 + g.dt * fu + ou[e]) * 0.5;
-                const double vn = (n.v000 + g.dt * fv + ov[e]) * 0.5;
-                ou[e] = un;
-                ov[e] = vn;
-                ow[e] = (n.w000 + g.dt * fw + ow[e]) * 0.5;
-                if (MEAN) {
-                    const double wk = g.dzflzi[k];
-                    if (fmask & 1) su = su + un * wk;
-                    if (fmask & 2) sv = sv + vn * wk;
-                }
-            } else {
-                ou[e] = n.u000 + g.dt * fu;
-                ov[e] = n.v000 + g.dt * fv;
-                ow[e] = n.w000 + g.dt * fw;
-                if (CORR) {
-                    cr.nu[e] = n.u000;
-                    cr.nv[e] = n.v000;
-                    cr.nw[e] = n.w000;
-                }
-            }
-            u00m = n.u000;
-            v00m = n.v000;
-            w00m = n.w000;
-            wp0m = n.wp00;
-            w0pm = n.w0p0;
+    // plane k from LDS buffer b (plane k+1 in b ^ 1); old: RK2's previous
+    // values of plane k when loaded ahead (FULL), else read here
+    auto compute = [&](int kv, int b, long long ek, const double *old) {
+        // k is the same for the whole workgroup: the z metrics come through
+        // the scalar unit (scalar loads wait on their own counter)
+        const int k = __builtin_amdgcn_readfirstlane(kv);
+        const double *cu = tl[b][0], *cv = tl[b][1], *cw = tl[b][2];
+        const double *nu = tl[b ^ 1][0], *nv = tl[b ^ 1][1], *nw = tl[b ^ 1][2];
+        Nb n;
+        n.u000 = cu[c0]; n.up00 = cu[c0 + 1]; n.um00 = cu[c0 - 1];
+        n.u0p0 = cu[c0 + LTX]; n.u0m0 = cu[c0 - LTX]; n.ump0 = cu[c0 - 1 + LTX];
+        n.u00p = nu[c0]; n.um0p = nu[c0 - 1]; n.u00m = u00m;
+        n.v000 = cv[c0]; n.vp00 = cv[c0 + 1]; n.vm00 = cv[c0 - 1];
+        n.v0p0 = cv[c0 + LTX]; n.v0m0 = cv[c0 - LTX]; n.vpm0 = cv[c0 + 1 - LTX];
+        n.v00p = nv[c0]; n.v0mp = nv[c0 - LTX]; n.v00m = v00m;
+        n.w000 = cw[c0]; n.wp00 = cw[c0 + 1]; n.wm00 = cw[c0 - 1];
+        n.w0p0 = cw[c0 + LTX]; n.w0m0 = cw[c0 - LTX]; n.w00p = nw[c0];
+        n.w00m = w00m; n.wp0m = wp0m; n.w0pm = w0pm;
+        double fu, fv, fw;
+        mom_rhs(g, n, k, fu, fv, fw);
+        if (BODY) {
+            fu = fu + fb.u[ek];
+            fv = fv + fb.v[ek];
+            fw = fw + fb.w[ek];
         }
-        __syncthreads();  // plane k's buffer is free
-        if (more) put(b);  // it becomes plane k+2
-        __syncthreads();
+        if (SECOND) {
+            const double ou0 = old ? old[0] : ou[ek], ov0 = old ? old[1] : ov[ek];
+            const double ow0 = old ? old[2] : ow[ek];
+            const double un = (n.u000 + g.dt * fu + ou0) * 0.5;
+            const double vn = (n.v000 + g.dt * fv + ov0) * 0.5;
+            ou[ek] = un;
+            ov[ek] = vn;
+            ow[ek] = (n.w000 + g.dt * fw + ow0) * 0.5;
+            if (MEAN) {
+                // both sums are formed (no branch in the loop); a component
+                // not in fmask reports 0
+                const double wk = pll::zm(g.dzflzi, k);
+                su = su + un * wk;
+                sv = sv + vn * wk;
+            }
+        } else {
+            ou[ek] = n.u000 + g.dt * fu;
+            ov[ek] = n.v000 + g.dt * fv;
+            ow[ek] = n.w000 + g.dt * fw;
+            if (CORR) {
+                cr.nu[ek] = n.u000;
+                cr.nv[ek] = n.v000;
+                cr.nw[ek] = n.w000;
+            }
+        }
+        u00m = n.u000;
+        v00m = n.v000;
+        w00m = n.w000;
+        wp0m = n.wp00;
+        w0pm = n.w0p0;
+    };
+    if constexpr (FULL) {
+        double oldc[3] = {0.0, 0.0, 0.0}, oldn[3];
+        if (SECOND) {
+            oldc[0] = ou[e];
+            oldc[1] = ov[e];
+            oldc[2] = ow[e];
+        }
+        // every load of the prologue has arrived before the loop starts, so that
+        // no wait inside the loop (which covers its first iteration too) has
+        // to include the loads in flight at that point
+        asm volatile("" : : "v"(u00m), "v"(v00m), "v"(w00m), "v"(wp0m), "v"(w0pm), "v"(oldc[0]),
+                     "v"(oldc[1]), "v"(oldc[2]));
+        auto step = [&](int k, auto INTERIOR) {
+            const int b = (k - k0) & 1;
+            if (SECOND) {
+                const long long en = k + 1 <= k1 ? e + sz : e;
+                oldn[0] = ou[en];
+                oldn[1] = ov[en];
+                oldn[2] = ow[en];
+            }
+            fetch(min(k + 2, k1 + 1), decltype(INTERIOR)::value);
+            // keep the prefetch ahead of the arithmetic (the scheduler would
+            // otherwise sink part of it next to its use)
+            __builtin_amdgcn_sched_barrier(0);
+            compute(k, b, e, SECOND ? oldc : nullptr);
+            __syncthreads();                      // plane k's buffer is free
+            put(b, decltype(INTERIOR)::value);    // it becomes plane k+2 (the last
+            __syncthreads();                      // two: a plane not read)
+            if (SECOND) {
+                oldc[0] = oldn[0];
+                oldc[1] = oldn[1];
+                oldc[2] = oldn[2];
+            }
+            e += sz;
+        };
+        // planes whose plane k+2 lies inside take the put without the wall
+        // rules: a wall rule in the loop would let the compiler move the w
+        // prefetch into its branch, next to its use
+        const int kin = min(k1, g.nz - 3);
+        int k = k0;
+#pragma unroll 1
+        for (; k <= kin; ++k) step(k, std::true_type{});
+#pragma unroll 1
+        for (; k <= k1; ++k) step(k, std::false_type{});
+    } else {
+        for (int k = k0; k <= k1; ++k, e += sz) {
+            const int b = (k - k0) & 1;
+            const bool more = k + 2 <= k1 + 1;
+            if (more) fetch(k + 2);
+            if (valid) compute(k, b, e, nullptr);
+            __syncthreads();  // plane k's buffer is free
+            if (more) put(b);  // it becomes plane k+2
+            __syncthreads();
+        }
     }
     if (MEAN) {
         __shared__ double red[2][NT];
@@ -390,8 +461,8 @@ __global__ void __launch_bounds__(RK_TX *TY) k_rk(G g, Tiles t, const double *__
             __syncthreads();
         }
         if (tid == 0) {
-            mpart[blockIdx.x] = red[0][0];
-            mpart[gridDim.x + blockIdx.x] = red[1][0];
+            mpart[blockIdx.x] = (fmask & 1) ? red[0][0] : 0.0;
+            mpart[gridDim.x + blockIdx.x] = (fmask & 2) ? red[1][0] : 0.0;
         }
     }
 }
@@ -725,6 +796,20 @@ int with_flags(bool a, bool b, bool c, F &&f)
     return -1;
 }
 
+// Grids whose RK tiles all lie inside take the FULL kernels for RK1 and the
+// projecting RK1 (768x384x256, same-process A/B: projecting RK1 1.698 ->
+// 1.542 ms, RK1 unchanged); RK2's FULL form needs 145 VGPRs (3 waves per SIMD
+// against 4) and runs 1.314 ms against 1.304, so RK2 keeps the general form
+// unless PLL_RK2_FULL=1.  PLL_RK_FULL=0 takes the general kernels everywhere.
+// (read at every launch, so that one process can time both forms)
+bool full_tiles(const G &g, int ty, bool second = false)
+{
+    const char *e = getenv("PLL_RK_FULL");
+    const char *e2 = getenv("PLL_RK2_FULL");
+    const bool allow = !(e && atoi(e) == 0), allow2 = e2 && atoi(e2) != 0;
+    return allow && (!second || allow2) && g.nx % RK_TX == 0 && g.ny % ty == 0;
+}
+
 extern "C" int pll_launch_rk1(const pll_grid_dev *gd, const double *u, const double *v,
                               const double *w, double *u1, double *v1, double *w1,
                               const double *const *body, hipStream_t s)
@@ -733,9 +818,10 @@ extern "C" int pll_launch_rk1(const pll_grid_dev *gd, const double *u, const dou
     const Tiles t = rk_tiles(g);
     const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
     const Body fb = body ? Body{body[0], body[1], body[2]} : Body{nullptr, nullptr, nullptr};
-    return with_flags(false, body != nullptr, false, [&](auto, auto B, auto) {
-        hipLaunchKernelGGL((k_rk<false, false, decltype(B)::value>), grid, blk, 0, s, g, t, u, v,
-                           w, u1, v1, w1, nullptr, 0, fb);
+    return with_flags(false, body != nullptr, full_tiles(g, RK_TY), [&](auto, auto B, auto F) {
+        hipLaunchKernelGGL((k_rk<false, false, decltype(B)::value, false, RK_TY, false,
+                                 decltype(F)::value>),
+                           grid, blk, 0, s, g, t, u, v, w, u1, v1, w1, nullptr, 0, fb, Corr{});
         return ok();
     });
 }
@@ -759,8 +845,14 @@ extern "C" int pll_launch_rk1_corr(const pll_grid_dev *gd, const double *u, cons
     cr.vfu = vfu;
     cr.vfv = vfv;
     cr.mean = mean;
-    hipLaunchKernelGGL((k_rk<false, false, false, true, RK_TY_CORR>), grid, blk, 0, s, g, t, u, v, w, u1, v1,
-                       w1, nullptr, 0, Body{nullptr, nullptr, nullptr}, cr);
+    if (full_tiles(g, RK_TY_CORR))
+        hipLaunchKernelGGL((k_rk<false, false, false, true, RK_TY_CORR, false, true>), grid, blk, 0,
+                           s, g, t, u, v, w, u1, v1, w1, nullptr, 0,
+                           Body{nullptr, nullptr, nullptr}, cr);
+    else
+        hipLaunchKernelGGL((k_rk<false, false, false, true, RK_TY_CORR, false, false>), grid, blk,
+                           0, s, g, t, u, v, w, u1, v1, w1, nullptr, 0,
+                           Body{nullptr, nullptr, nullptr}, cr);
     return ok();
 }
 
@@ -775,10 +867,16 @@ extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, doub
     const Body fb = body ? Body{body[0], body[1], body[2]} : Body{nullptr, nullptr, nullptr};
     const bool mean = mpart && fmask;
     return with_flags(mean, body != nullptr, fill != 0, [&](auto M, auto B, auto F) {
-        hipLaunchKernelGGL((k_rk<true, decltype(M)::value, decltype(B)::value, false, RK_TY,
-                                 decltype(F)::value>),
-                           grid, blk, 0, s, g, t, u1, v1, w1, u, v, w, mean ? mpart : nullptr,
-                           mean ? fmask : 0, fb, Corr{});
+        if (full_tiles(g, RK_TY, true))
+            hipLaunchKernelGGL((k_rk<true, decltype(M)::value, decltype(B)::value, false, RK_TY,
+                                     decltype(F)::value, true>),
+                               grid, blk, 0, s, g, t, u1, v1, w1, u, v, w, mean ? mpart : nullptr,
+                               mean ? fmask : 0, fb, Corr{});
+        else
+            hipLaunchKernelGGL((k_rk<true, decltype(M)::value, decltype(B)::value, false, RK_TY,
+                                     decltype(F)::value, false>),
+                               grid, blk, 0, s, g, t, u1, v1, w1, u, v, w, mean ? mpart : nullptr,
+                               mean ? fmask : 0, fb, Corr{});
         return ok();
     });
 }
