@@ -550,25 +550,40 @@ __device__ __forceinline__ void mirror(const V (&v)[E], int t, int T, int n, uns
 
 // ---------------------------------------------------------------- x I/O
 // Sources of the x forward transform: real row (j,k) of nx values.
+// kLow: the source also gives the FP64 values (load_d), from which the x
+// forward transform of the refined form projects the lowest modes in FP64.
 struct SrcHalo64 {  // interior of a haloed FP64 array (the caller's p / rhs)
+    static constexpr bool kLow = true;
     const double *p;
     long long off1, off2;
-    template <typename V>
-    __device__ __forceinline__ V load(long long, long long j, long long k, int m, int) const
+    __device__ __forceinline__ double2 load_d(long long, long long j, long long k, int m, int) const
     {
         const d2u pr = *reinterpret_cast<const d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m);
+        return make_double2(pr.x, pr.y);
+    }
+    template <typename V>
+    __device__ __forceinline__ V load(long long row, long long j, long long k, int m, int nx) const
+    {
+        const double2 pr = load_d(row, j, k, m, nx);
         return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
     }
 };
 struct SrcHalo64Copy {  // SrcHalo64, also copying the FP64 values to a dense array
+    static constexpr bool kLow = true;
     const double *p;
     long long off1, off2;
     double *F;
-    template <typename V>
-    __device__ __forceinline__ V load(long long row, long long j, long long k, int m, int nx) const
+    __device__ __forceinline__ double2 load_d(long long row, long long j, long long k, int m,
+                                              int nx) const
     {
         const d2u pr = *reinterpret_cast<const d2u *>(p + (k + 1) * off1 + (j + 1) * off2 + 1 + 2 * m);
         *reinterpret_cast<double2 *>(F + row * nx + 2 * m) = make_double2(pr.x, pr.y);
+        return make_double2(pr.x, pr.y);
+    }
+    template <typename V>
+    __device__ __forceinline__ V load(long long row, long long j, long long k, int m, int nx) const
+    {
+        const double2 pr = load_d(row, j, k, m, nx);
         return mk<V>((typename Sc<V>::T)pr.x, (typename Sc<V>::T)pr.y);
     }
 };
@@ -577,13 +592,20 @@ struct SrcHalo64Copy {  // SrcHalo64, also copying the FP64 values to a dense ar
 // rounded to the transform's precision.  f and p are each haloed or dense
 // (element (i,j,k) at base + k s1 + j s2 + i).
 struct SrcResid {
+    static constexpr bool kLow = true;
     const double *f, *P;
     long long fb, f1, f2, pb, p1, p2;
     const double *a, *c, *d;
     double idx2, idy2;
     int ny, nz;
     template <typename V>
-    __device__ __forceinline__ V load(long long, long long j, long long k, int m, int nx) const
+    __device__ __forceinline__ V load(long long row, long long j, long long k, int m, int nx) const
+    {
+        const double2 r = load_d(row, j, k, m, nx);
+        return mk<V>((typename Sc<V>::T)(float)r.x, (typename Sc<V>::T)(float)r.y);
+    }
+    __device__ __forceinline__ double2 load_d(long long, long long j, long long k, int m,
+                                              int nx) const
     {
         const int i0 = 2 * m, i1 = i0 + 1;
         const int im = i0 == 0 ? nx - 1 : i0 - 1, ip = i1 + 1 == nx ? 0 : i1 + 1;
@@ -620,11 +642,12 @@ struct SrcResid {
         const double lz0 = ak * (zm0 - c0) + ck * (zp0 - c0) + dk * c0;
         const double lz1 = ak * (zm1 - c1) + ck * (zp1 - c1) + dk * c1;
         const double r0 = f0 - (lx0 + ly0 + lz0), r1 = fv1 - (lx1 + ly1 + lz1);
-        return mk<V>((typename Sc<V>::T)(float)r0, (typename Sc<V>::T)(float)r1);
+        return make_double2(r0, r1);
     }
 };
 
 struct SrcDense32 {  // dense FP32 rows (the residual of a refinement sweep)
+    static constexpr bool kLow = false;
     const float *r;
     template <typename V>
     __device__ __forceinline__ V load(long long row, long long, long long, int m, int nx) const
@@ -635,6 +658,7 @@ struct SrcDense32 {  // dense FP32 rows (the residual of a refinement sweep)
 };
 
 struct SrcDiv {  // PPE right-hand side formed from the velocity on the fly
+    static constexpr bool kLow = false;
     pll_div_src q;
     long long off1, off2;
     template <typename V>
@@ -657,6 +681,7 @@ struct SrcDiv {  // PPE right-hand side formed from the velocity on the fly
     }
 };
 struct SrcDivUnfilled {  // SrcDiv on a velocity whose boundary fill is not made
+    static constexpr bool kLow = false;
     pll_div_src q;
     long long off1, off2;
     int nx, ny, nz;
@@ -826,6 +851,13 @@ struct XArgs {
     int nx, ny, rows_per_wg, pitch;
     double norm;
     double *norm_part;  // NORM kernels: [2 * workgroups] (max |value|, max |result|)
+    // refined form, lowest modes in FP64 (null: off).  x forward: low_rx[2 row
+    // + kx] = sum_i x_i exp(-2 pi i kx i/nx), kx = 0, 1, of the FP64 row; x
+    // backward: low_x[6 k + q] the solved modes of plane k, q = 3 kx + (0, 1,
+    // 2 for ky = 0, 1, ny - 1), added to the row (normalised)
+    double2 *low_rx;
+    const double2 *low_x;
+    const double2 *twx64, *twy64;  // exp(-2 pi i t/nx), exp(-2 pi i t/ny) in FP64
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin; renumbering them so
@@ -882,10 +914,70 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
     unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
     V v[E];
+    if constexpr (sizeof(S) == sizeof(float) && Src::kLow) {
+        if (a.low_rx) {
+            // FP64 sums of the row for kx = 0 and 1: each FP64 value x is
+            // split into its FP32 rounding v (the transform's input) and the
+            // rounding error x - v; the errors (at most 2^-24 |x|) are summed
+            // in FP32 as they are loaded, the v afterwards in FP64, so that the
+            // FP64 values of the row segment need not stay in registers.
+            // exp(-2 pi i x/nx) at x = 2m, 2m + 1 by recurrences (m steps by T).
+            float e0 = 0.0f, e1r = 0.0f, e1i = 0.0f;
+            {
+                const double2 a0 = a.twx64[2 * t], a1 = a.twx64[2 * t + 1];
+                const double2 as = a.twx64[(2 * T) % a.nx];
+                float2 u0 = make_float2((float)a0.x, (float)a0.y);
+                float2 u1 = make_float2((float)a1.x, (float)a1.y);
+                const float2 us = make_float2((float)as.x, (float)as.y);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int m = t + e * T;
-        v[e] = valid ? src.template load<V>(row, j, k, m, a.nx) : mk<V>(S(0), S(0));
+                for (int e = 0; e < E; ++e) {
+                    const int m = t + e * T;
+                    double2 d = make_double2(0.0, 0.0);
+                    if (valid) d = src.load_d(row, j, k, m, a.nx);
+                    v[e] = mk<V>((S)d.x, (S)d.y);
+                    const float r0 = (float)(d.x - (double)v[e].x);
+                    const float r1 = (float)(d.y - (double)v[e].y);
+                    e0 += r0 + r1;
+                    e1r += r0 * u0.x + r1 * u1.x;
+                    e1i += r0 * u0.y + r1 * u1.y;
+                    u0 = make_float2(u0.x * us.x - u0.y * us.y, u0.x * us.y + u0.y * us.x);
+                    u1 = make_float2(u1.x * us.x - u1.y * us.y, u1.x * us.y + u1.y * us.x);
+                }
+            }
+            double2 w0 = a.twx64[2 * t], w1 = a.twx64[2 * t + 1];
+            const double2 st = a.twx64[(2 * T) % a.nx];
+            double s0 = (double)e0, s1r = (double)e1r, s1i = (double)e1i;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double x0 = (double)v[e].x, x1 = (double)v[e].y;
+                s0 += x0 + x1;
+                s1r += x0 * w0.x + x1 * w1.x;
+                s1i += x0 * w0.y + x1 * w1.y;
+                w0 = make_double2(w0.x * st.x - w0.y * st.y, w0.x * st.y + w0.y * st.x);
+                w1 = make_double2(w1.x * st.x - w1.y * st.y, w1.x * st.y + w1.y * st.x);
+            }
+            for (int o = T >> 1; o > 0; o >>= 1) {
+                s0 += __shfl_xor(s0, o);
+                s1r += __shfl_xor(s1r, o);
+                s1i += __shfl_xor(s1i, o);
+            }
+            if (valid && t == 0) {
+                a.low_rx[2 * row] = make_double2(s0, 0.0);
+                a.low_rx[2 * row + 1] = make_double2(s1r, s1i);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int m = t + e * T;
+                v[e] = valid ? src.template load<V>(row, j, k, m, a.nx) : mk<V>(S(0), S(0));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            v[e] = valid ? src.template load<V>(row, j, k, m, a.nx) : mk<V>(S(0), S(0));
+        }
     }
     F::template run<-1, 2>(v, t, a.P, a.tw, ml);
     __syncthreads();
@@ -972,12 +1064,46 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
                 if (m == L - 1) dst.row_one(j, k, 2 * L, ls[m]);
             }
         }
-    } else if constexpr (!NORM) {
+    } else {
+    // refined form, lowest modes solved in FP64: their contribution to row
+    // (j, k) is norm (B + 2 Re(A exp(2 pi i x/nx))), with
+    // B = X(0,0) + 2 Re(X(0,1) e), A = X(1,0) + X(1,1) e + X(1,-1) conj(e),
+    // e = exp(2 pi i j/ny); the FP32 pipeline carries none of these modes
+    double lb = 0.0, lar = 0.0, lai = 0.0;
+    const bool low = sizeof(S) == sizeof(float) && a.low_x != nullptr;
+    if (low && valid) {
+        const double2 *X = a.low_x + 6 * k;
+        const double2 ty = a.twy64[j];
+        const double ec = ty.x, es = -ty.y;  // e = exp(+2 pi i j/ny)
+        const double2 x00 = X[0], x01 = X[1], x10 = X[3], x11 = X[4], x1m = X[5];
+        lb = (x00.x + 2.0 * (x01.x * ec - x01.y * es)) * a.norm;
+        lar = (x10.x + (x11.x * ec - x11.y * es) + (x1m.x * ec + x1m.y * es)) * a.norm;
+        lai = (x10.y + (x11.x * es + x11.y * ec) + (x1m.y * ec - x1m.x * es)) * a.norm;
+    }
+    // exp(-2 pi i x/nx) (cos, -sin) at x = 2m, 2m + 1, advanced by a
+    // recurrence as m steps by T
+    double2 tw0 = make_double2(1.0, 0.0), tw1 = tw0, tst = tw0;
+    if (low) {
+        tw0 = a.twx64[2 * t];
+        tw1 = a.twx64[2 * t + 1];
+        tst = a.twx64[(2 * T) % a.nx];
+    }
+    double lc0 = 0.0, lc1 = 0.0;  // contributions at 2m, 2m + 1 of the current e
+    auto lowstep = [&]() {
+        if (!low) return;
+        lc0 = lb + 2.0 * (lar * tw0.x + lai * tw0.y);
+        lc1 = lb + 2.0 * (lar * tw1.x + lai * tw1.y);
+        tw0 = make_double2(tw0.x * tst.x - tw0.y * tst.y, tw0.x * tst.y + tw0.y * tst.x);
+        tw1 = make_double2(tw1.x * tst.x - tw1.y * tst.y, tw1.x * tst.y + tw1.y * tst.x);
+    };
+    if constexpr (!NORM) {
         if (valid) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int m = t + e * T;
-                dst.store(row, j, k, m, a.nx, (double)v[e].x * a.norm, (double)v[e].y * a.norm);
+                lowstep();
+                dst.store(row, j, k, m, a.nx, (double)v[e].x * a.norm + lc0,
+                          (double)v[e].y * a.norm + lc1);
             }
         }
     } else {
@@ -986,7 +1112,9 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int m = t + e * T;
-                const double x0 = (double)v[e].x * a.norm, x1 = (double)v[e].y * a.norm;
+                lowstep();
+                const double x0 = (double)v[e].x * a.norm + lc0;
+                const double x1 = (double)v[e].y * a.norm + lc1;
                 np = nmax(np, dst.store_n(row, j, k, m, a.nx, x0, x1));
                 nc = nmax(nc, nmax(fabs(x0), fabs(x1)));
             }
@@ -1012,6 +1140,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
             a.norm_part[2 * blockIdx.x] = nc;
             a.norm_part[2 * blockIdx.x + 1] = np;
         }
+    }
     }
 }
 
@@ -1150,6 +1279,10 @@ XArgs<V> x_args(const pll_ppe_dev *d, const void *tw)
     a.tw = (const V *)tw;
     a.row0 = 0;
     a.xcd = 0;
+    a.low_rx = nullptr;
+    a.low_x = nullptr;
+    a.twx64 = (const double2 *)d->tw_x;
+    a.twy64 = (const double2 *)d->tw_y;
     a.nrows = (long long)d->ny * d->nz;
     a.nx = d->nx;
     a.ny = d->ny;
@@ -1175,6 +1308,10 @@ int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
     if (threads > max_threads_f<F, V>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     a.xcd = d->x_xcd >= 2 || (d->x_xcd == 1 && sizeof(V) == sizeof(float2));
+    if (sizeof(V) == sizeof(float2) && d->low_on) {
+        if (!Src::kLow) return -1;
+        a.low_rx = (double2 *)d->low_rx;
+    }
     hipLaunchKernelGGL((k_xfwd<F, V, Src>), dim3((unsigned)blocks), dim3(threads), shm, s, src,
                        (V *)W, a.tw, a);
     return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -1195,6 +1332,7 @@ int launch_xb(const pll_ppe_dev *d, const void *tw, const void *W, Dst dst, hipS
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
     if (threads > max_threads_f<F, V>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     a.xcd = d->x_xcd >= 2 || (d->x_xcd == 1 && sizeof(V) == sizeof(float2));
+    if (sizeof(V) == sizeof(float2) && d->low_on) a.low_x = (const double2 *)d->low_x;
     if (NORM) {
         // norm[0..1]: result; norm[2 ..): partials, room for ny*nz workgroups
         if (!norm || blocks > a.nrows || shm < sizeof(double) * 2 * ((threads + 63) / 64))

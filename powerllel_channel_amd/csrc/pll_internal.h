@@ -60,6 +60,12 @@ typedef struct {
         const double *lamy;    /* [ny/2+1] */
         const int *order;      /* [tiles] launch order, exact tiles first */
     } tp;
+    /* refined form: the lowest modes (kx in {0,1}, ky in {0,1,ny-1}) of each
+     * FP32 solve in FP64 (low_on = 0: off).  low_rx [ny nz][2] the x sums of
+     * the FP64 rows, low_y [nz][6] their y sums, low_x [nz][6] the solved
+     * modes (double2 each) */
+    int low_on;
+    void *low_rx, *low_y, *low_x;
 } pll_ppe_dev;
 
 int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, hipStream_t s);
@@ -121,6 +127,12 @@ int pll_launch_resid_hp(const pll_ppe_dev *d, const double *F, const double *p_h
 int pll_launch_resid(const pll_ppe_dev *d, const double *f_halo, const double *P, float *r,
                      double *gauge, hipStream_t s);
 int pll_gauge_parts(const pll_ppe_dev *d);
+/* Lowest modes of the refined form in FP64, after an x forward that wrote
+ * low_rx: their y sums, their tridiagonal solves (the reference's recurrence
+ * with the stored pivots and the pinned gauge) into low_x; and the same
+ * modes set to zero in the FP32 spectrum W32 (after its y forward). */
+int pll_launch_low_solve(const pll_ppe_dev *d, hipStream_t s);
+int pll_launch_low_zero(const pll_ppe_dev *d, void *W32, hipStream_t s);
 int pll_launch_yfwd(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s);
 int pll_launch_ybwd(const pll_ppe_dev *d, void *W, hipStream_t s);

@@ -179,6 +179,7 @@ struct pll_ppe {
     void *bz, *lamx, *lamy, *order;   /* partitioned z solve */
     void *tw_x32, *tw_y32, *W32, *P64, *r32; /* refined form (allocated on first use) */
     void *gauge;                      /* top-plane partial sums + mean (refined form) */
+    void *low_rx, *low_y, *low_x;     /* lowest modes in FP64 (refined form) */
     int sweeps;                       /* pll_ppe_solve: -1 FP64, >= 0 FP32 + sweeps */
     double tol;                       /* > 0: pll_ppe_solve refines to this tolerance,
                                          with at most `sweeps` sweeps */
@@ -217,7 +218,8 @@ int pll_ppe_destroy(pll_ppe *h)
     if (!h) return PLL_OK;
     void *bufs[] = {h->W, h->tw_x, h->tw_y, h->a, h->c, h->piv, h->stage,
                     h->bz, h->lamx, h->lamy, h->order, h->tw_x32, h->tw_y32,
-                    h->W32, h->P64, h->r32, h->gauge, h->nrm};
+                    h->W32, h->P64, h->r32, h->gauge, h->nrm, h->low_rx, h->low_y,
+                    h->low_x};
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
         if (bufs[i]) hipFree(bufs[i]);
     if (h->timing)
@@ -688,6 +690,9 @@ static int ensure_fp32(pll_ppe *h)
         (!h->P64 && (rc = dmalloc(h, &h->P64, sizeof(double) * N))) ||
         (!h->r32 && (rc = dmalloc(h, &h->r32, sizeof(float) * N))) ||
         (!h->gauge && (rc = dmalloc(h, &h->gauge, sizeof(double) * (size_t)(pll_gauge_parts(&h->d) + 1)))) ||
+        (!h->low_rx && (rc = dmalloc(h, &h->low_rx, 32 * (size_t)h->ny * h->nz))) ||
+        (!h->low_y && (rc = dmalloc(h, &h->low_y, 96 * (size_t)h->nz))) ||
+        (!h->low_x && (rc = dmalloc(h, &h->low_x, 96 * (size_t)h->nz))) ||
         (rc = dmalloc(h, &h->W32, sizeof(float) * N))) {
         free(tx); free(ty);
         return rc;
@@ -703,16 +708,38 @@ static int ensure_fp32(pll_ppe *h)
     }
     h->d.tw_x32 = h->tw_x32;
     h->d.tw_y32 = h->tw_y32;
+    h->d.low_rx = h->low_rx;
+    h->d.low_y = h->low_y;
+    h->d.low_x = h->low_x;
     return PLL_OK;
+}
+
+/* The lowest modes of every FP32 solve in FP64 (pll_launch_low_solve): with
+ * the residual formed in the x transform (the projection needs its FP64
+ * rows), a row of the x transform inside one wavefront (its lane count a
+ * power of two <= 64) and ny >= 4; PLL_REFINE_LOW=0 turns it off. */
+static int low_modes_ok(const pll_ppe *h, int fused)
+{
+    const char *pl = getenv("PLL_REFINE_LOW");
+    if (pl && atoi(pl) == 0) return 0;
+    const int T = h->d.px.T;
+    return fused && T >= 1 && T <= 64 && (T & (T - 1)) == 0 && h->ny >= 4 && h->L >= 2 &&
+           21 * 8 * (size_t)h->nz <= 163840;
 }
 
 static int spectral32(pll_ppe *h, hipStream_t s)
 {
     int rc;
+    if (h->d.low_on && (rc = launch_rc(pll_launch_low_solve(&h->d, s), "lowest modes (fp64)")))
+        return rc;
     if ((rc = launch_rc(pll_launch_yfwd32(&h->d, h->W32, s), "y forward (fp32)"))) return rc;
+    if (h->d.low_on && (rc = launch_rc(pll_launch_low_zero(&h->d, h->W32, s), "lowest modes")))
+        return rc;
     if ((rc = launch_rc(pll_launch_trid32(&h->d, h->W32, s), "z solve (fp32 storage)"))) return rc;
     return launch_rc(pll_launch_ybwd32(&h->d, h->W32, s), "y backward (fp32)");
 }
+
+static int refined_sweeps(pll_ppe *h, double *d_var, int sweeps, int fused, hipStream_t s);
 
 int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream)
 {
@@ -722,6 +749,17 @@ int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream)
     int rc;
     if ((rc = ensure_fp32(h))) return rc;
     hipStream_t s = (hipStream_t)stream;
+    const char *pf = getenv("PLL_REFINE_FUSED");
+    const int fused = !(pf && atoi(pf) == 0);
+    h->d.low_on = low_modes_ok(h, fused);
+    rc = refined_sweeps(h, d_var, sweeps, fused, s);
+    h->d.low_on = 0;
+    return rc;
+}
+
+static int refined_sweeps(pll_ppe *h, double *d_var, int sweeps, int fused, hipStream_t s)
+{
+    int rc;
     if ((rc = launch_rc(pll_launch_xfwd32(&h->d, PLL_SRC_HALO64, d_var, h->W32, s), "x forward (fp32)")))
         return rc;
     if ((rc = spectral32(h, s))) return rc;
@@ -737,8 +775,6 @@ int pll_ppe_solve_refined(pll_ppe *h, double *d_var, int sweeps, void *stream)
      * sweep measures the top-plane mean of P and the final write removes it. */
     double *gauge = h->d.pin_gauge ? (double *)h->gauge : NULL;
     const double *eps = gauge ? gauge + pll_gauge_parts(&h->d) : NULL;
-    const char *pf = getenv("PLL_REFINE_FUSED");
-    const int fused = !(pf && atoi(pf) == 0);
     for (int it = 1; it <= sweeps; it++) {
         const int last = it == sweeps;
         if (fused) {
@@ -809,6 +845,9 @@ static int read_norms(pll_ppe *h, hipStream_t s, double out[2])
     return PLL_OK;
 }
 
+static int refined_tol(pll_ppe *h, double *d_var, double tol, int max_sweeps, int fused,
+                       hipStream_t s);
+
 int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_sweeps,
                               int *sweeps_done, double *err_est, void *stream)
 {
@@ -818,7 +857,21 @@ int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_swe
         return pll_set_error(PLL_EINVAL, "need tol > 0 and max_sweeps in [1, 64]");
     int rc;
     if ((rc = ensure_fp32(h)) || (rc = ensure_norm(h))) return rc;
-    hipStream_t s = (hipStream_t)stream;
+    const char *pf = getenv("PLL_REFINE_FUSED");
+    const int fused = !(pf && atoi(pf) == 0);
+    h->d.low_on = low_modes_ok(h, fused);
+    rc = refined_tol(h, d_var, tol, max_sweeps, fused, (hipStream_t)stream);
+    h->d.low_on = 0;
+    if (rc) return rc;
+    if (sweeps_done) *sweeps_done = h->last_sweeps;
+    if (err_est) *err_est = h->last_est;
+    return PLL_OK;
+}
+
+static int refined_tol(pll_ppe *h, double *d_var, double tol, int max_sweeps, int fused,
+                       hipStream_t s)
+{
+    int rc;
     double *F = (double *)h->P64, *nrm = (double *)h->nrm;
     if ((rc = launch_rc(pll_launch_xfwd32_copy(&h->d, d_var, F, h->W32, s), "x forward (fp32)")))
         return rc;
@@ -832,8 +885,6 @@ int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_swe
     double *gauge = h->d.pin_gauge ? (double *)h->gauge : NULL;
     const double *eps = gauge ? gauge + pll_gauge_parts(&h->d) : NULL;
     int it;
-    const char *pf = getenv("PLL_REFINE_FUSED");
-    const int fused = !(pf && atoi(pf) == 0);
     for (it = 1; it <= max_sweeps; it++) {
         if (fused) {
             if (gauge && (rc = launch_rc(pll_launch_gauge(&h->d, d_var, 1, gauge, s), "gauge")))
@@ -863,8 +914,6 @@ int pll_ppe_solve_refined_tol(pll_ppe *h, double *d_var, double tol, int max_swe
     }
     h->last_sweeps = it <= max_sweeps ? it : max_sweeps;
     h->last_est = est;
-    if (sweeps_done) *sweeps_done = h->last_sweeps;
-    if (err_est) *err_est = est;
     return PLL_OK;
 }
 

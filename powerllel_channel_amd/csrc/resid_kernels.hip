@@ -212,3 +212,146 @@ extern "C" int pll_launch_gauge(const pll_ppe_dev *d, const double *P, int halo,
                        1.0 / ((double)d->nx * d->ny), gauge + n);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
+
+// ------------------------------------------------------------ lowest modes
+// The FP32 transforms of a refinement sweep carry an absolute error of about
+// 6e-8 |r| into every spectral coefficient; the lowest (kx, ky) modes have
+// the smallest eigenvalues, so their solution is the least accurate in
+// relative terms, and the error left after one sweep sits there.  These modes
+// (kx = 0, 1 and ky = 0, 1, ny - 1) are therefore projected from the FP64
+// rows in the x forward transform (low_rx), summed over y and solved here in
+// FP64, removed from the FP32 spectrum, and added back in the x backward
+// transform.  768x384x256, CPU emulation of the FP32 pipeline: error after
+// one sweep 4.4e-11 with all modes in FP32, 7.7e-12 with these six in FP64.
+namespace {
+
+// plane k: low_y[6k + 3 kx + q] = sum_j low_rx[2 (k ny + j) + kx] exp(-2 pi i ky j / ny),
+// ky = 0, 1, ny - 1 for q = 0, 1, 2; one workgroup per plane, fixed order
+__global__ void __launch_bounds__(256) k_low_y(const double2 *__restrict__ rx,
+                                               const double2 *__restrict__ twy,
+                                               double2 *__restrict__ ly, int ny)
+{
+    __shared__ double red[12][256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    double acc[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) acc[q] = 0.0;
+    for (int j = t; j < ny; j += 256) {
+        const double2 w = twy[j];  // exp(-2 pi i j/ny)
+#pragma unroll
+        for (int kx = 0; kx < 2; ++kx) {
+            const double2 r = rx[2 * ((long long)k * ny + j) + kx];
+            acc[6 * kx + 0] += r.x;  // ky = 0
+            acc[6 * kx + 1] += r.y;
+            acc[6 * kx + 2] += r.x * w.x - r.y * w.y;  // ky = 1: r w
+            acc[6 * kx + 3] += r.x * w.y + r.y * w.x;
+            acc[6 * kx + 4] += r.x * w.x + r.y * w.y;  // ky = ny - 1: r conj(w)
+            acc[6 * kx + 5] += r.y * w.x - r.x * w.y;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) red[q][t] = acc[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) red[q][t] += red[q][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 6) ly[6 * (long long)k + t] = make_double2(red[2 * t][0], red[2 * t + 1][0]);
+}
+
+// The twelve real systems (mode q = 0..5, real and imaginary part) in one
+// workgroup: the multipliers m_k = a_k / piv_{k-1} and 1 / piv_k of the four
+// pivot columns (kx, |ky|) in {0,1}^2 are formed in parallel into LDS with
+// the right-hand sides; then one lane per system runs the recurrences of
+// ssolve_trid (src/src_c/mod_poisson_solver.c:319-356) with the stored
+// pivots, forward y_k = d_k - m_k y_{k-1}, top x = y / piv (0 for a zero
+// pivot and for the pinned (0,0) mode), backward x_k = (y_k - c_k x_{k+1}) / piv_k,
+// each step one multiply-add on LDS values.
+__global__ void __launch_bounds__(256) k_low_z(const double2 *__restrict__ ly,
+                                               double2 *__restrict__ lx,
+                                               const double *__restrict__ piv,
+                                               const double *__restrict__ a,
+                                               const double *__restrict__ c, int L, int ny, int nz,
+                                               int pin_gauge)
+{
+    extern __shared__ double sm[];
+    double *rhs = sm;                 // [nz][12]: (q, part) = 2 q + part
+    double *mk = rhs + 12 * nz;       // [4][nz]  a_k / piv_{k-1}
+    double *ip = mk + 4 * nz;         // [4][nz]  1 / piv_k
+    double *cc = ip + 4 * nz;         // [nz]
+    const long long pst = (long long)(L + 1) * (ny / 2 + 1);
+    const int t = threadIdx.x;
+    for (int e = t; e < 12 * nz; e += blockDim.x)
+        rhs[e] = reinterpret_cast<const double *>(ly)[e];
+    for (int e = t; e < 4 * nz; e += blockDim.x) {
+        const int cls = e / nz, k = e - cls * nz, kx = cls & 1, iy = cls >> 1;
+        const double *pc = piv + (long long)iy * (L + 1) + kx;
+        const double pk = pc[(long long)k * pst];
+        mk[e] = k > 0 ? a[k] / pc[(long long)(k - 1) * pst] : 0.0;
+        ip[e] = pk != 0.0 ? 1.0 / pk : 0.0;
+    }
+    for (int k = t; k < nz; k += blockDim.x) cc[k] = c[k];
+    __syncthreads();
+    if (t >= 12) return;
+    const int q = t >> 1, kx = q / 3, kyq = q % 3;
+    const int cls = kx + 2 * (kyq == 0 ? 0 : 1);
+    const double *m = mk + cls * nz, *r = ip + cls * nz;
+    double prev = rhs[t];
+    for (int k = 1; k < nz; ++k) {
+        prev = rhs[12 * k + t] - m[k] * prev;
+        rhs[12 * k + t] = prev;
+    }
+    const bool pin = pin_gauge && q == 0;
+    double next = pin ? 0.0 : prev * r[nz - 1];  // r = 0 for a zero pivot
+    double *dst = reinterpret_cast<double *>(lx) + t;
+    dst[12 * (long long)(nz - 1)] = next;
+    for (int k = nz - 2; k >= 0; --k) {
+        next = (rhs[12 * k + t] - cc[k] * next) * r[k];
+        dst[12 * (long long)k] = next;
+    }
+}
+
+// W32 after its y forward: column 0 slot 0 real part (A_0) and slot 1 (A_1);
+// column 1 slots 0, 1, ny - 1
+__global__ void __launch_bounds__(256) k_low_zero(float2 *__restrict__ W, int L, int ny, int nz)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nz) return;
+    float2 *P = W + (long long)k * ny * L;
+    P[0].x = 0.0f;
+    P[L] = make_float2(0.0f, 0.0f);
+    P[1] = make_float2(0.0f, 0.0f);
+    P[L + 1] = make_float2(0.0f, 0.0f);
+    P[(long long)(ny - 1) * L + 1] = make_float2(0.0f, 0.0f);
+}
+
+}  // namespace
+
+extern "C" int pll_launch_low_solve(const pll_ppe_dev *d, hipStream_t s)
+{
+    if (!d->low_on || d->ny < 4 || d->L < 2) return -1;
+    hipLaunchKernelGGL(k_low_y, dim3(d->nz), dim3(256), 0, s, (const double2 *)d->low_rx,
+                       (const double2 *)d->tw_y, (double2 *)d->low_y, d->ny);
+    if (hipGetLastError() != hipSuccess) return -3;
+    const size_t shm = sizeof(double) * 21 * (size_t)d->nz;
+    if (shm > 163840) return -1;
+    if (shm > 65536 && hipFuncSetAttribute((const void *)k_low_z,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)shm) != hipSuccess)
+        return -3;
+    hipLaunchKernelGGL(k_low_z, dim3(1), dim3(256), shm, s, (const double2 *)d->low_y,
+                       (double2 *)d->low_x, d->piv, d->a, d->c, d->L, d->ny, d->nz, d->pin_gauge);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+extern "C" int pll_launch_low_zero(const pll_ppe_dev *d, void *W32, hipStream_t s)
+{
+    if (!d->low_on) return -1;
+    hipLaunchKernelGGL(k_low_zero, dim3((d->nz + 255) / 256), dim3(256), 0, s, (float2 *)W32, d->L,
+                       d->ny, d->nz);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
