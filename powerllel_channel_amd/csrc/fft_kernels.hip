@@ -593,6 +593,7 @@ struct SrcHalo64Copy {  // SrcHalo64, also copying the FP64 values to a dense ar
 // (element (i,j,k) at base + k s1 + j s2 + i).
 struct SrcResid {
     static constexpr bool kLow = true;
+    static constexpr bool kStencil = true;
     const double *f, *P;
     long long fb, f1, f2, pb, p1, p2;
     const double *a, *c, *d;
@@ -847,6 +848,7 @@ struct XArgs {
     const V *tw;        // [nx] exp(-2 pi i t/nx)
     long long row0;     // first row of this launch (0: the whole array)
     int xcd;            // number workgroups per XCD (neighbouring rows share an L2)
+    int slab;           // xcd with slabs: workgroups per plane in one XCD's slab (0: none)
     long long nrows;    // one past its last row (ny*nz: the whole array)
     int nx, ny, rows_per_wg, pitch;
     double norm;
@@ -869,6 +871,36 @@ __device__ __forceinline__ unsigned xcd_block()
     const unsigned b = blockIdx.x, n = gridDim.x;
     return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
 }
+
+// First row of an x kernel's workgroup (row = k ny + j).  With a.slab, XCD x
+// (= blockIdx & 7) takes the rows j in [x ny/8, (x+1) ny/8) of every plane,
+// plane after plane: a stencil source reads rows j +- 1 and planes k +- 1 of
+// each row, and those were read by the same XCD a slab (ny/8 rows) earlier,
+// so they are still in its L2.  A contiguous range of rows per XCD puts a
+// whole plane (about 6 MB of traffic at ny = 384) between a row and its z
+// neighbours, more than the 4 MB L2; round-robin splits the y neighbours
+// over XCDs.  Needs row0 = 0, nrows = ny nz and ny % (8 rows_per_wg) = 0.
+template <typename V>
+__device__ __forceinline__ long long x_row_first(const XArgs<V> &a)
+{
+    if (a.slab) {
+        const unsigned b = blockIdx.x, x = b & 7, p = b >> 3;
+        const unsigned k = p / (unsigned)a.slab, q = p - k * (unsigned)a.slab;
+        return (long long)k * a.ny + (long long)(x * (unsigned)a.slab + q) * a.rows_per_wg;
+    }
+    return a.row0 + (long long)(a.xcd ? xcd_block() : blockIdx.x) * a.rows_per_wg;
+}
+
+// Sources that read neighbouring rows and planes (stencils) take the slab
+// order (Src::kStencil); the others keep contiguous ranges.
+template <class S, class = void>
+struct is_stencil {
+    static constexpr bool value = false;
+};
+template <class S>
+struct is_stencil<S, decltype((void)S::kStencil)> {
+    static constexpr bool value = S::kStencil;
+};
 
 template <typename V>
 struct YArgs {
@@ -909,7 +941,7 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
-    const long long row = a.row0 + (long long)(a.xcd ? xcd_block() : blockIdx.x) * a.rows_per_wg + rl;
+    const long long row = x_row_first(a) + rl;
     const bool valid = row < a.nrows;
     unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
@@ -1014,7 +1046,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
     constexpr int E = F::E;
     const int T = F::threads(a.P), L = F::length(a.P);
     const int rl = threadIdx.x / T, t = threadIdx.x - rl * T;
-    const long long row = a.row0 + (long long)(a.xcd ? xcd_block() : blockIdx.x) * a.rows_per_wg + rl;
+    const long long row = x_row_first(a) + rl;
     const bool valid = row < a.nrows;
     unsigned char *ml = lds_raw + (size_t)rl * a.pitch * lds_elem<F, V>();
     const long long j = valid ? row % a.ny : 0, k = valid ? row / a.ny : 0;
@@ -1279,6 +1311,7 @@ XArgs<V> x_args(const pll_ppe_dev *d, const void *tw)
     a.tw = (const V *)tw;
     a.row0 = 0;
     a.xcd = 0;
+    a.slab = 0;
     a.low_rx = nullptr;
     a.low_x = nullptr;
     a.twx64 = (const double2 *)d->tw_x;
@@ -1308,6 +1341,9 @@ int launch_xf(const pll_ppe_dev *d, const void *tw, Src src, void *W, hipStream_
     const size_t shm = (size_t)a.rows_per_wg * a.pitch * lds_elem<F, V>();
     if (threads > max_threads_f<F, V>() || shm > 65536 || blocks > 0x7fffffffLL) return -1;
     a.xcd = d->x_xcd >= 2 || (d->x_xcd == 1 && sizeof(V) == sizeof(float2));
+    if (a.xcd && d->x_slab && (is_stencil<Src>::value || d->x_slab >= 2) && a.row0 == 0 &&
+        a.nrows == (long long)d->ny * d->nz && d->ny % (8 * a.rows_per_wg) == 0)
+        a.slab = d->ny / (8 * a.rows_per_wg);
     if (sizeof(V) == sizeof(float2) && d->low_on) {
         if (!Src::kLow) return -1;
         a.low_rx = (double2 *)d->low_rx;
@@ -1464,10 +1500,18 @@ extern "C" int pll_launch_xfwd(const pll_ppe_dev *d, const double *p, void *W, h
     });
 }
 
+// The chunked transforms of host-pointer solves are their own instantiations
+// (same code), so that kernel traces keep them apart from whole-array ones.
+struct SrcHalo64Rows : SrcHalo64 {};
+struct DstHalo64Rows : DstHalo64 {};
+
 extern "C" int pll_launch_xfwd_rows(const pll_ppe_dev *d, const double *p, void *W,
                                     long long row0, long long row1, hipStream_t s)
 {
-    const SrcHalo64 src{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    SrcHalo64Rows src;
+    src.p = p;
+    src.off1 = (long long)(d->nx + 2) * (d->ny + 2);
+    src.off2 = d->nx + 2;
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xf<decltype(f), double2>(d, d->tw_x, src, W, s, row0, row1);
     });
@@ -1500,7 +1544,10 @@ extern "C" int pll_launch_xbwd(const pll_ppe_dev *d, const void *W, double *p, h
 extern "C" int pll_launch_xbwd_rows(const pll_ppe_dev *d, const void *W, double *p,
                                     long long row0, long long row1, hipStream_t s)
 {
-    const DstHalo64 dst{p, (long long)(d->nx + 2) * (d->ny + 2), d->nx + 2};
+    DstHalo64Rows dst;
+    dst.p = p;
+    dst.off1 = (long long)(d->nx + 2) * (d->ny + 2);
+    dst.off2 = d->nx + 2;
     return dispatch_plan(d->px, [&](auto f) {
         return launch_xb<decltype(f), double2>(d, d->tw_x, W, dst, s, nullptr, row0, row1);
     });

@@ -42,6 +42,7 @@ typedef struct {
     int y_cols_per_wg;         /* columns per workgroup, y kernels  */
     int y_cols32_per_wg;       /* the same for the FP32 y kernels   */
     int x_xcd;                 /* x kernels numbered per XCD: 0 none, 1 FP32, 2 all */
+    int x_slab;                /* ... in y slabs per XCD: 0 none, 1 stencil sources, 2 all */
     const void *tw_x;          /* double2[nx]: exp(-2 pi i t / nx)  */
     const void *tw_y;          /* double2[ny]: exp(-2 pi i t / ny)  */
     const void *tw_x32;        /* float2[nx], same table in FP32 (refined form) */

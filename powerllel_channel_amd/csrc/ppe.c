@@ -286,6 +286,11 @@ static int choose_launch(pll_ppe *h)
      * PLL_X_XCD: 0 none, 1 FP32 kernels (default), 2 all */
     const char *px_ = getenv("PLL_X_XCD");
     h->d.x_xcd = px_ ? atoi(px_) : 1;
+    /* and for the residual's stencil, each XCD in its own y slab of every
+     * plane, so that the rows' y and z neighbours are still in its L2.
+     * PLL_X_SLAB: 0 none, 1 stencil sources (default), 2 all per-XCD x kernels */
+    const char *ps_ = getenv("PLL_X_SLAB");
+    h->d.x_slab = ps_ ? atoi(ps_) : 1;
     return PLL_OK;
 }
 

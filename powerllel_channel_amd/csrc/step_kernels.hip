@@ -107,8 +107,8 @@ __device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n
 // fetched again instead of skipped, so the loop has no divergent branch and
 // each wait covers only the loads it needs: the prefetch is waited for when it
 // is written to LDS (the plane's stores may stay in flight), and RK2's old
-// values of plane k+1 are loaded before the prefetch of plane k+2, so they
-// have arrived when plane k+1 is computed.
+// values of plane k are loaded before the prefetch of plane k+2, so that
+// waiting for them leaves the prefetch in flight.
 template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY, bool FILL = false,
           bool FULL = false>
 __global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : 1)) k_rk(G g, Tiles t, const double *__restrict__ qu,
@@ -394,24 +394,19 @@ __global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : 1)) k_rk(G
         w0pm = n.w0p0;
     };
     if constexpr (FULL) {
-        double oldc[3] = {0.0, 0.0, 0.0}, oldn[3];
-        if (SECOND) {
-            oldc[0] = ou[e];
-            oldc[1] = ov[e];
-            oldc[2] = ow[e];
-        }
+        double oldc[3] = {0.0, 0.0, 0.0};
         // every load of the prologue has arrived before the loop starts, so that
         // no wait inside the loop (which covers its first iteration too) has
         // to include the loads in flight at that point
-        asm volatile("" : : "v"(u00m), "v"(v00m), "v"(w00m), "v"(wp0m), "v"(w0pm), "v"(oldc[0]),
-                     "v"(oldc[1]), "v"(oldc[2]));
+        asm volatile("" : : "v"(u00m), "v"(v00m), "v"(w00m), "v"(wp0m), "v"(w0pm));
         auto step = [&](int k, auto INTERIOR) {
             const int b = (k - k0) & 1;
             if (SECOND) {
-                const long long en = k + 1 <= k1 ? e + sz : e;
-                oldn[0] = ou[en];
-                oldn[1] = ov[en];
-                oldn[2] = ow[en];
+                // RK2's old values of plane k, issued before the prefetch so
+                // that waiting for them leaves the prefetch in flight
+                oldc[0] = ou[e];
+                oldc[1] = ov[e];
+                oldc[2] = ow[e];
             }
             fetch(min(k + 2, k1 + 1), decltype(INTERIOR)::value);
             // keep the prefetch ahead of the arithmetic (the scheduler would
@@ -421,11 +416,6 @@ __global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : 1)) k_rk(G
             __syncthreads();                      // plane k's buffer is free
             put(b, decltype(INTERIOR)::value);    // it becomes plane k+2 (the last
             __syncthreads();                      // two: a plane not read)
-            if (SECOND) {
-                oldc[0] = oldn[0];
-                oldc[1] = oldn[1];
-                oldc[2] = oldn[2];
-            }
             e += sz;
         };
         // planes whose plane k+2 lies inside take the put without the wall

@@ -308,6 +308,40 @@ def test_refined_error_per_sweep(cuda, nx, ny, nz, sr):
         assert e <= REFINED_BAR[k], (k, e)
 
 
+@pytest.mark.parametrize("nx,ny,nz,sr", [(768, 384, 16, 4.0), (256, 128, 24, 3.0)])
+def test_x_row_orders_bitwise(cuda, nx, ny, nz, sr, monkeypatch):
+    """The x transforms' workgroup-to-row orders (round-robin over XCDs,
+    contiguous row ranges per XCD, y slabs per XCD) only change which
+    workgroup does a row: refined solves (residual stencil, lowest modes in
+    FP64) must agree bit for bit."""
+    import torch
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, spectral_rhs
+    print(f"[x row orders] {nx}x{ny}x{nz}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, sr)
+    rhs = spectral_rhs(g, 3)
+    out = {}
+    for env in ({}, {"PLL_X_SLAB": "0"}, {"PLL_X_SLAB": "2"}, {"PLL_X_XCD": "0"},
+                {"PLL_X_XCD": "2"}):
+        for k in ("PLL_X_SLAB", "PLL_X_XCD"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+        p = torch.from_numpy(rhs).to(cuda)
+        s.solve_refined(p, 1)
+        q = torch.from_numpy(rhs).to(cuda)
+        s.solve(q)
+        torch.cuda.synchronize()
+        out[str(env)] = (p, q)
+        s.close()
+    p0, q0 = out["{}"]
+    for key, (p, q) in out.items():
+        assert torch.equal(p, p0), key
+        assert torch.equal(q, q0), key
+
+
 def test_refined_low_modes_one_sweep(cuda, monkeypatch):
     """768x384x256, stretch 4: with the six lowest (kx, ky) modes of each FP32
     solve projected, solved and added back in FP64, one sweep reaches the

@@ -441,6 +441,52 @@ def test_run_equals_steps(orc, cuda, shape, forced, u_crf, nsteps, monkeypatch):
     ch.close()
 
 
+@pytest.mark.parametrize("env", [{"PLL_RK_FULL": "0"}, {"PLL_RK2_FULL": "1"}])
+@pytest.mark.parametrize("u_crf", [0.0, 0.37])
+def test_rk_tile_forms_bitwise(orc, cuda, env, u_crf, monkeypatch):
+    """The step kernels have a general form (bounds-checked tiles) and a
+    full-tile form for grids whose x and y extents are multiples of the tile;
+    both must give the oracle's stencils bit for bit: RK1 and RK2 on their
+    own, then a forced run (Mean_h reduced inside RK2, projection inside
+    RK1), against the default selection."""
+    import torch
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.workloads import poiseuille_fields
+    nx, ny, nz = 128, 48, 24
+    print(f"[rk tile forms] {env} u_crf={u_crf}", flush=True)
+    g, G, ch, (u, v, w, u1, v1, w1, _) = _setup(orc, nx, ny, nz, 3.0, u_crf=u_crf)
+    for k, val in env.items():
+        monkeypatch.setenv(k, val)
+    du, dv, dw, du1, dv1, dw1 = _dev(torch, cuda, u, v, w, u1, v1, w1)
+    ch.rk1(du, dv, dw, du1, dv1, dw1)
+    orc.rk1(G, u, v, w, u1, v1, w1, u_crf)
+    ch.rk2(du, dv, dw, du1, dv1, dw1)
+    orc.rk2(G, u, v, w, u1, v1, w1, u_crf)
+    torch.cuda.synchronize()
+    for a, b in zip(_host(du1, dv1, dw1, du, dv, dw), (u1, v1, w1, u, v, w)):
+        assert np.array_equal(a, b)
+    ch.close()
+    monkeypatch.setenv("PLL_RUN_CHAIN", "1")
+    ch = Channel(nx, ny, nz, g.lx, g.ly, g.lz, g.dzf, 1 / 5600, 1e-3, u_crf=u_crf,
+                 is_forced=(True, True, False), vel_force=(1.0 - u_crf, 0.25, 0.0))
+    s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+    u, v, w = poiseuille_fields(g, 24)
+    u -= u_crf
+    z = np.zeros(g.shape)
+    a = _dev(torch, cuda, u, v, w, z, z, z, z)
+    ch.bc_vel(*a[:3])
+    b = [t.clone() for t in a]
+    ch.run(s, 4, *a)
+    for k in env:
+        monkeypatch.delenv(k)
+    ch.run(s, 4, *b)
+    torch.cuda.synchronize()
+    for name, x, y in zip("u v w".split(), a[:3], b[:3]):
+        assert torch.equal(x, y), (name, float((x - y).abs().max()))
+    s.close()
+    ch.close()
+
+
 @pytest.mark.parametrize("chain", ["1", "0"])
 def test_run_calls_vs_steps(orc, cuda, chain, monkeypatch):
     """Four pll_channel_run calls in a row (5, 3, 7 and 5 steps), with
