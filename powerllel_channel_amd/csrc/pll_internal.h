@@ -60,6 +60,9 @@ typedef struct {
         const double *lamx;    /* [L+1]    */
         const double *lamy;    /* [ny/2+1] */
         const int *order;      /* [tiles] launch order, exact tiles first */
+        int h32;               /* FP32 storage: 2 = tiles of 2 S columns (0/1: S) */
+        int qx32;              /* qx for those tiles */
+        const int *order32;    /* [nx/(2S) ny] their launch order */
     } tp;
     /* refined form: the lowest modes (kx in {0,1}, ky in {0,1,ny-1}) of each
      * FP32 solve in FP64 (low_on = 0: off).  low_rx [ny nz][2] the x sums of

@@ -176,7 +176,7 @@ struct pll_ppe {
     int nx, ny, nz, L;
     pll_ppe_dev d;
     void *W, *tw_x, *tw_y, *a, *c, *piv;
-    void *bz, *lamx, *lamy, *order;   /* partitioned z solve */
+    void *bz, *lamx, *lamy, *order, *order32;  /* partitioned z solve */
     void *tw_x32, *tw_y32, *W32, *P64, *r32; /* refined form (allocated on first use) */
     void *gauge;                      /* top-plane partial sums + mean (refined form) */
     void *low_rx, *low_y, *low_x;     /* lowest modes in FP64 (refined form) */
@@ -219,7 +219,7 @@ int pll_ppe_destroy(pll_ppe *h)
     void *bufs[] = {h->W, h->tw_x, h->tw_y, h->a, h->c, h->piv, h->stage,
                     h->bz, h->lamx, h->lamy, h->order, h->tw_x32, h->tw_y32,
                     h->W32, h->P64, h->r32, h->gauge, h->nrm, h->low_rx, h->low_y,
-                    h->low_x};
+                    h->low_x, h->order32};
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
         if (bufs[i]) hipFree(bufs[i]);
     if (h->timing)
@@ -333,6 +333,39 @@ static int choose_part(int nx, int nz, int *M, int *P, int *S)
     for (size_t i = 0; i < sizeof Ms / sizeof Ms[0]; i++)
         if (part_fits(nx, nz, Ms[i], M, P, S)) return 1;
     return 0;
+}
+
+/* Launch order of the z-solve tiles (SW columns each, nx/SW per y row): tiles
+ * with exact-path columns (q0 < qx, iy < iyx) first; workgroup b runs tile
+ * order[b].  Workgroups reach the 8 XCDs round-robin, so each XCD is handed a
+ * contiguous run of the list: neighbouring tiles (sharing 128-byte lines in
+ * FP32 storage) then meet in one L2 (PLL_TRID_XCD=0 keeps the plain order).
+ * Returns a malloc'd array, NULL when out of memory. */
+static int *tile_order(int nx, int ny, int SW, int qx, int iyx)
+{
+    const int ntx = nx / SW, ntiles = ntx * ny;
+    int *order = (int *)malloc(sizeof(int) * (size_t)ntiles);
+    if (!order) return NULL;
+    int n = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int t = 0; t < ntiles; t++) {
+            const int my = t / ntx, q0 = (t % ntx) * SW;
+            const int iy = my <= ny - my ? my : ny - my;
+            const int ex = (q0 < qx && iy < iyx);
+            if (ex == (pass == 0)) order[n++] = t;
+        }
+    const char *px = getenv("PLL_TRID_XCD");
+    if (ntiles % 8 == 0 && !(px && atoi(px) == 0)) {
+        int *perm = (int *)malloc(sizeof(int) * (size_t)ntiles);
+        if (!perm) {
+            free(order);
+            return NULL;
+        }
+        for (int b = 0; b < ntiles; b++) perm[b] = order[(b % 8) * (ntiles / 8) + b / 8];
+        free(order);
+        order = perm;
+    }
+    return order;
 }
 
 /* Eigenvalue magnitude below which a column is solved with the sequential
@@ -467,36 +500,40 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
             int ixmax = -1;
             while (ixmax + 1 <= L && fabs(lx[ixmax + 1]) < thr) ixmax++;
             int qx = iyx > 0 ? 2 * (ixmax + 1) : 0;
-            qx = ((qx + S - 1) / S) * S;
-            int *order = (int *)malloc(sizeof(int) * (size_t)ntiles);
+            const int qx0 = qx;
+            qx = ((qx0 + S - 1) / S) * S;
+            int *order = tile_order(nx, ny, S, qx, iyx);
             if (!order) {
                 rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
                 goto fail;
             }
-            int n = 0;
-            for (int pass = 0; pass < 2; pass++)
-                for (int t = 0; t < ntiles; t++) {
-                    const int my = t / ntx, q0 = (t % ntx) * S;
-                    const int iy = my <= ny - my ? my : ny - my;
-                    const int ex = (q0 < qx && iy < iyx);
-                    if (ex == (pass == 0)) order[n++] = t;
-                }
-            /* workgroup b runs tile order[b]; workgroups reach the 8 XCDs
-             * round-robin, so hand each XCD a contiguous run of this list:
-             * neighbouring tiles (sharing 128-byte lines in FP32 storage)
-             * then meet in one L2 */
-            const char *px = getenv("PLL_TRID_XCD"); /* tuning: 0 keeps the plain order */
-            if (ntiles % 8 == 0 && !(px && atoi(px) == 0)) {
-                int *perm = (int *)malloc(sizeof(int) * (size_t)ntiles);
-                if (!perm) {
+            /* FP32 storage: tiles of 2 S columns (256-byte row segments at
+             * S = 32), if their LDS fits (PLL_TRID32_H=1: S columns) */
+            const char *ph = getenv("PLL_TRID32_H");
+            const int h32 = (S >= 16 && nx % (2 * S) == 0 &&
+                             (size_t)nz * 2 * S * 8 + (size_t)nz * 24 + (size_t)P * 8 <= 163840 &&
+                             !(ph && atoi(ph) == 1)) ? 2 : 1;
+            const int qx32 = ((qx0 + 2 * S - 1) / (2 * S)) * (2 * S);
+            if (h32 == 2) {
+                int *order32 = tile_order(nx, ny, 2 * S, qx32, iyx);
+                if (!order32 || (rc = dmalloc(h, &h->order32, sizeof(int) * (size_t)(ntiles / 2)))) {
+                    free(order32);
                     free(order);
-                    rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
+                    if (!rc) rc = pll_set_error(PLL_ENOMEM, "host allocation failed");
                     goto fail;
                 }
-                for (int b = 0; b < ntiles; b++) perm[b] = order[(b % 8) * (ntiles / 8) + b / 8];
-                free(order);
-                order = perm;
+                const hipError_t e32 = hipMemcpy(h->order32, order32, sizeof(int) * (size_t)(ntiles / 2),
+                                                 hipMemcpyHostToDevice);
+                free(order32);
+                if (e32 != hipSuccess) {
+                    free(order);
+                    rc = pll_set_error(PLL_EHIP, "copy of z-solve tables failed");
+                    goto fail;
+                }
             }
+            h->d.tp.h32 = h32;
+            h->d.tp.qx32 = qx32;
+            h->d.tp.order32 = (const int *)h->order32;
             double *lxh = (double *)malloc(sizeof(double) * (size_t)(L + 1));
             if (!lxh) {
                 free(order);

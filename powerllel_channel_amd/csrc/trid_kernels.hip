@@ -164,9 +164,12 @@ __device__ __forceinline__ void pcr16_stage(double &fa, double &fg, double &fd, 
     fa = fa2; fg = fg2; fd = fd2; la = la2; lg = lg2; ld = ld2;
 }
 
-__device__ __forceinline__ int swz(int k, int s, int S, int M)
+// LDS slot of row k, column c of a tile SW = S H columns wide: the column
+// within its group of S is swizzled by the row's block k / M, so that the P
+// lanes of a column (rows p M + i) fall in distinct banks.
+__device__ __forceinline__ int swz(int k, int c, int S, int SW, int M)
 {
-    return k * S + (s ^ ((k / M) & (S - 1)));
+    return k * SW + (c & ~(S - 1)) + ((c & (S - 1)) ^ ((k / M) & (S - 1)));
 }
 
 __device__ __forceinline__ double shfl(double v, int src)
@@ -192,14 +195,20 @@ __device__ __forceinline__ int lam_ix(int q, int my, int L, int ny)
 // XCD t % 8.
 // DPP16: the P = 16 lanes of a column form one DPP row; the cyclic reduction
 // then exchanges neighbours with DPP instead of ds_bpermute + select.
-template <int M, int S, typename St, bool DPP16>
+// H > 1 (FP32 storage): a tile is H groups of S columns, SW = S H columns
+// wide in memory (256-byte row segments where S floats make only 128), and
+// the S P threads solve the groups one after the other; g.ntx counts tiles of
+// SW columns.
+template <int M, int S, typename St, bool DPP16, int H = 1>
 __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
                                                    const int *__restrict__ order, PArgs g)
 {
     extern __shared__ double tl[];
     const int lP = g.lP, P = 1 << lP, nz = g.nz;
-    constexpr int LS = S == 32 ? 5 : (S == 16 ? 4 : (S == 8 ? 3 : 2));
+    constexpr int SW = S * H;
+    constexpr int LSW = SW == 64 ? 6 : (SW == 32 ? 5 : (SW == 16 ? 4 : (SW == 8 ? 3 : 2)));
     static_assert(S == 4 || S == 8 || S == 16 || S == 32, "columns per tile");
+    static_assert(H == 1 || H == 2, "column groups per tile");
     const int tid = threadIdx.x;
     const long long plane = (long long)g.nx * g.ny;
     const int nthr = S * P;
@@ -208,7 +217,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
     // CS = 3M + 1 puts the P lanes of a column in distinct banks, and a lane's
     // reads are at fixed offsets from one address.
     constexpr int CS = 3 * M + 1;
-    double *cf = tl + (size_t)nz * S;
+    double *cf = tl + (size_t)nz * SW;
 
     for (int k = tid; k < nz; k += nthr) {
         const int o = (k / M) * CS + 3 * (k % M);
@@ -216,23 +225,23 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
         cf[o + 1] = g.c[k];
         cf[o + 2] = g.bz[k];
     }
-    // element j of this thread: column st, row kt + j P (nthr = S P), at
-    // tile_base + toff + j jstride
-    const int st = tid & (S - 1), kt = tid >> LS;
-    const long long toff = st + kt * plane, jstride = (long long)P * plane;
-    St pre[M];
+    // element j of this thread: column st, row kt + j P/H (nthr = SW P/H),
+    // at tile_base + toff + j jstride
+    const int lPH = lP - (H == 2), st = tid & (SW - 1), kt = tid >> LSW;
+    const long long toff = st + kt * plane, jstride = (long long)(P / H) * plane;
+    St pre[H * M];
     auto fetch = [&](int tile) {
         const St *__restrict__ src =
-            Wr + (long long)(tile / g.ntx) * g.nx + (tile % g.ntx) * S + toff;
+            Wr + (long long)(tile / g.ntx) * g.nx + (tile % g.ntx) * SW + toff;
 #pragma unroll
-        for (int j = 0; j < M; ++j) pre[j] = src[j * jstride];
+        for (int j = 0; j < H * M; ++j) pre[j] = src[j * jstride];
     };
     int tile = (int)blockIdx.x < g.ntiles ? order[blockIdx.x] : 0;
     if ((int)blockIdx.x < g.ntiles) fetch(tile);
     // Results of the previous tile, held in registers until the current
     // tile's values are in LDS: stores and loads share one in-order counter,
     // so a store issued before that wait would be waited for with it.
-    St out[M];
+    St out[H * M];
     St *__restrict__ out_base = nullptr;
     // Every global load of an iteration is issued before the prefetch of the
     // next tile, and the prefetch is unconditional (the last iteration
@@ -241,26 +250,33 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
     for (int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
     const int tn = t + (int)gridDim.x;
     const int tile_next = order[tn < g.ntiles ? tn : t];
-    const int my = tile / g.ntx, q0 = (tile % g.ntx) * S;
-    St *__restrict__ base = Wr + (long long)my * g.nx + q0;
+    const int my = tile / g.ntx, qt0 = (tile % g.ntx) * SW;
+    St *__restrict__ base = Wr + (long long)my * g.nx + qt0;
     const int iy = my <= g.ny - my ? my : g.ny - my;
     const int s = tid >> lP, p = tid & (P - 1);
-    const int q = q0 + s;
-    const double lam = g.lamx[lam_ix(q, my, g.L, g.ny)] + g.lamy[iy];
+    double lams[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h)
+        lams[h] = g.lamx[lam_ix(qt0 + h * S + s, my, g.L, g.ny)] + g.lamy[iy];
     __syncthreads();  // the previous tile's results have been read from LDS
 #pragma unroll
-    for (int j = 0; j < M; ++j) tl[swz(kt + (j << lP), st, S, M)] = (double)pre[j];
+    for (int j = 0; j < H * M; ++j) tl[swz(kt + (j << lPH), st, S, SW, M)] = (double)pre[j];
     __syncthreads();
     // lam is needed from here on: waiting for it now, before the stores and
     // the prefetch are issued, keeps those out of that wait
-    asm volatile("" : : "v"(lam));
+#pragma unroll
+    for (int h = 0; h < H; ++h) asm volatile("" : : "v"(lams[h]));
     if (out_base) {
 #pragma unroll
-        for (int j = 0; j < M; ++j) out_base[j * jstride] = out[j];
+        for (int j = 0; j < H * M; ++j) out_base[j * jstride] = out[j];
     }
     fetch(tile_next);
 
-    if (q0 < g.qx && iy < g.iyx) {
+#pragma unroll 1
+    for (int h = 0; h < H; ++h) {
+    const int q = qt0 + h * S + s;
+    const double lam = (H == 1 || h == 0) ? lams[0] : lams[H - 1];
+    if (qt0 < g.qx && iy < g.iyx) {
         // The reference's recurrences with the stored pivots (low modes, where
         // the block elimination below loses accuracy):
         //   forward  y_k = d_k - m_k y_{k-1},  m_k = a_k / piv_{k-1},
@@ -270,7 +286,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
         // over the P lanes of the column gives every lane its entering value,
         // and the lane then runs the recurrence over its rows as written.
         const int k0 = p * M;
-        const int pb = k0 * S + (s ^ (p & (S - 1)));
+        const int pb = k0 * SW + h * S + (s ^ (p & (S - 1)));
         const int lane = tid & 63;
         const long long pst = (long long)(g.L + 1) * (g.ny / 2 + 1);
         const double *__restrict__ pc =
@@ -285,7 +301,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double m = (k0 + i > 0) ? cl[3 * i] / (i ? pv[i - 1] : pleft) : 0.0;
-            al = tl[pb + i * S] - m * al;
+            al = tl[pb + i * SW] - m * al;
             be = -m * be;
         }
         for (int t = 1; t < P; t *= 2) {
@@ -301,8 +317,8 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double m = (k0 + i > 0) ? cl[3 * i] / (i ? pv[i - 1] : pleft) : 0.0;
-            const double yi = tl[pb + i * S] - m * prev;
-            tl[pb + i * S] = yi;
+            const double yi = tl[pb + i * SW] - m * prev;
+            tl[pb + i * SW] = yi;
             prev = yi;
         }
         // top row: pinned gauge, y / piv, or 0 for an exactly zero pivot
@@ -318,7 +334,7 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
                 de = 0.0;
             } else {
                 const double c = cl[3 * i + 1];
-                ga = (tl[pb + i * S] - c * ga) / pv[i];
+                ga = (tl[pb + i * SW] - c * ga) / pv[i];
                 de = -(c * de) / pv[i];
             }
         }
@@ -334,17 +350,17 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
 #pragma unroll
         for (int i = M - 1; i >= 0; --i) {
             const double x =
-                (top && i == M - 1) ? xtop : (tl[pb + i * S] - cl[3 * i + 1] * next) / pv[i];
-            tl[pb + i * S] = x;
+                (top && i == M - 1) ? xtop : (tl[pb + i * SW] - cl[3 * i + 1] * next) / pv[i];
+            tl[pb + i * SW] = x;
             next = x;
         }
     } else {
         const int k0 = p * M;
-        // swz(k0 + i, s) = pb + i S for i < M, since (k0 + i) / M = p
-        const int pb = k0 * S + (s ^ (p & (S - 1)));
+        // swz(k0 + i, h S + s) = pb + i SW for i < M, since (k0 + i) / M = p
+        const int pb = k0 * SW + h * S + (s ^ (p & (S - 1)));
         double d[M], ap[M], cp[M];
 #pragma unroll
-        for (int i = 0; i < M; ++i) d[i] = tl[pb + i * S];
+        for (int i = 0; i < M; ++i) d[i] = tl[pb + i * SW];
         // 1. local elimination
         const double *__restrict__ cl = cf + p * CS;
 #define CA(i) cl[3 * (i)]
@@ -452,17 +468,18 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
         d[0] = F;
         d[M - 1] = Lv;
 #pragma unroll
-        for (int i = 0; i < M; ++i) tl[pb + i * S] = d[i];
+        for (int i = 0; i < M; ++i) tl[pb + i * SW] = d[i];
     }
+    }  // h
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < M; ++j) out[j] = (St)tl[swz(kt + (j << lP), st, S, M)];
+    for (int j = 0; j < H * M; ++j) out[j] = (St)tl[swz(kt + (j << lPH), st, S, SW, M)];
     out_base = base + toff;
     tile = tile_next;
     }
     if (out_base) {
 #pragma unroll
-        for (int j = 0; j < M; ++j) out_base[j * jstride] = out[j];
+        for (int j = 0; j < H * M; ++j) out_base[j * jstride] = out[j];
     }
 }
 
@@ -495,6 +512,9 @@ template <typename St, typename C>
 int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
     if (d->tp.P > 0) {
+        // FP32 storage with wide tiles: H = 2 groups of S columns per tile
+        const int H = (sizeof(St) == sizeof(float) && d->tp.h32 == 2) ? 2 : 1;
+        constexpr int HW = sizeof(St) == sizeof(float) ? 2 : 1;  // H = 2 instantiated for FP32 only
         PArgs g;
         g.a = d->a;
         g.c = d->c;
@@ -502,7 +522,7 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         g.lamx = d->tp.lamx;
         g.lamy = d->tp.lamy;
         g.piv = d->piv;
-        g.order = d->tp.order;
+        g.order = H == 2 ? d->tp.order32 : d->tp.order;
         g.L = d->L;
         g.nx = d->nx;
         g.ny = d->ny;
@@ -512,33 +532,44 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         while ((1 << g.lP) < g.P) g.lP++;
         if ((1 << g.lP) != g.P) return -1;
         g.S = d->tp.S;
-        g.ntx = d->nx / d->tp.S;
+        if (d->nx % (g.S * H) || (H == 2 && (g.S < 16 || g.P < 2 || !g.order))) return -1;
+        g.ntx = d->nx / (g.S * H);
         g.ntiles = g.ntx * d->ny;
         g.pin_gauge = d->pin_gauge;
-        g.qx = d->tp.qx;
+        g.qx = H == 2 ? d->tp.qx32 : d->tp.qx;
         g.iyx = d->tp.iyx;
         const int threads = g.S * g.P;
         // values + coefficients (P rows of 3M + 1)
-        const size_t shm = sizeof(double) * ((size_t)d->nz * g.S + 3 * (size_t)d->nz + g.P);
+        const size_t shm = sizeof(double) * ((size_t)d->nz * g.S * H + 3 * (size_t)d->nz + g.P);
         if (threads > 512 || shm > 163840) return -1;
         switch (d->tp.M) {
-#define PLL_TRID_LAUNCH(M_, S_, D_)                                                             \
-    do {                                                                                        \
-        if (shm > 65536 && hipFuncSetAttribute((const void *)k_trid_part<M_, S_, St, D_>,       \
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                                               (int)shm) != hipSuccess)                         \
-            return -3;                                                                          \
-        const unsigned blocks =                                                                 \
-            persistent_grid((const void *)k_trid_part<M_, S_, St, D_>, threads, shm, g.ntiles); \
-        hipLaunchKernelGGL((k_trid_part<M_, S_, St, D_>), dim3(blocks), dim3(threads), shm, s,  \
-                           (St *)W, g.order, g);                                                \
+#define PLL_TRID_LAUNCH(M_, S_, D_, H_)                                                          \
+    do {                                                                                          \
+        if (shm > 65536 &&                                                                        \
+            hipFuncSetAttribute((const void *)k_trid_part<M_, S_, St, D_, H_>,                    \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) != hipSuccess) \
+            return -3;                                                                            \
+        const unsigned blocks = persistent_grid((const void *)k_trid_part<M_, S_, St, D_, H_>,   \
+                                                threads, shm, g.ntiles);                          \
+        hipLaunchKernelGGL((k_trid_part<M_, S_, St, D_, H_>), dim3(blocks), dim3(threads), shm, s, \
+                           (St *)W, g.order, g);                                                  \
     } while (0)
 #define PLL_TRID_S(M_, D_)                                                                      \
     switch (g.S) {                                                                              \
-        case 4: PLL_TRID_LAUNCH(M_, 4, D_); break;                                              \
-        case 8: PLL_TRID_LAUNCH(M_, 8, D_); break;                                              \
-        case 16: PLL_TRID_LAUNCH(M_, 16, D_); break;                                            \
-        case 32: PLL_TRID_LAUNCH(M_, 32, D_); break;                                            \
+        case 4: PLL_TRID_LAUNCH(M_, 4, D_, 1); break;                                           \
+        case 8: PLL_TRID_LAUNCH(M_, 8, D_, 1); break;                                           \
+        case 16:                                                                                \
+            if (H == 2)                                                                         \
+                PLL_TRID_LAUNCH(M_, 16, D_, HW);                                                \
+            else                                                                                \
+                PLL_TRID_LAUNCH(M_, 16, D_, 1);                                                 \
+            break;                                                                              \
+        case 32:                                                                                \
+            if (H == 2)                                                                         \
+                PLL_TRID_LAUNCH(M_, 32, D_, HW);                                                \
+            else                                                                                \
+                PLL_TRID_LAUNCH(M_, 32, D_, 1);                                                 \
+            break;                                                                              \
         default: return -5;                                                                     \
     }
 #define PLL_TRID_CASE(M_)                                                                       \

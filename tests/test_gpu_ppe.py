@@ -342,6 +342,39 @@ def test_x_row_orders_bitwise(cuda, nx, ny, nz, sr, monkeypatch):
         assert torch.equal(q, q0), key
 
 
+@pytest.mark.parametrize("nx,ny,nz,sr", [(768, 384, 16, 4.0), (256, 128, 24, 3.0),
+                                         (64, 48, 256, 4.0), (96, 64, 384, 4.4)])
+def test_fp32_wide_z_tiles(cuda, nx, ny, nz, sr, monkeypatch):
+    """FP32 z solves on tiles of 2 S columns (one workgroup solving two
+    column groups in turn) against tiles of S columns: the same per-column
+    arithmetic except that a wider first tile sends a few more low-mode
+    columns through the exact recurrence, so the FP32 solve agrees to FP32
+    rounding and the refined answers to the refinement's accuracy."""
+    import torch
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, spectral_rhs
+    print(f"[fp32 wide z tiles] {nx}x{ny}x{nz}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, sr)
+    rhs = spectral_rhs(g, 4)
+    inner = (slice(1, -1),) * 3
+    out = {}
+    for h in ("1", "2"):
+        monkeypatch.setenv("PLL_TRID32_H", h)
+        s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+        for k in (0, 2):
+            p = torch.from_numpy(rhs).to(cuda)
+            s.solve_refined(p, k)
+            torch.cuda.synchronize()
+            out[(h, k)] = p[inner].clone()
+        s.close()
+    scale = float(out[("1", 2)].abs().max())
+    e0 = float((out[("2", 0)] - out[("1", 0)]).abs().max()) / scale
+    e2 = float((out[("2", 2)] - out[("1", 2)]).abs().max()) / scale
+    print(f"  k=0 {e0:.2e}, k=2 {e2:.2e}", flush=True)
+    assert e0 <= 1e-6 and e2 <= 1e-11
+
+
 def test_refined_low_modes_one_sweep(cuda, monkeypatch):
     """768x384x256, stretch 4: with the six lowest (kx, ky) modes of each FP32
     solve projected, solved and added back in FP64, one sweep reaches the
