@@ -97,6 +97,12 @@ int pll_channel_create(pll_channel **out, int nx, int ny, int nz, double lx, dou
     c->g.re_inv = re_inv;
     c->g.dt = dt;
     c->g.u_crf = u_crf;
+    {
+        /* planes per k chunk of the stencil kernels' tiles (tuning override;
+         * fixed for the handle's life: the Mean_h partials are sized by it) */
+        const char *ek = getenv("PLL_RK_KC");
+        c->g.rk_kc = ek && atoi(ek) > 0 ? atoi(ek) : 0;
+    }
     for (int d = 0; d < 3; d++) {
         c->is_forced[d] = is_forced ? is_forced[d] != 0 : (d == 0);
         c->vel_force[d] = vel_force ? vel_force[d] : (d == 0 ? 1.0 : 0.0);

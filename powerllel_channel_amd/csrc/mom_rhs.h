@@ -15,6 +15,7 @@ struct G {
     double dx_inv, dy_inv, re_inv, dt, u_crf;
     const double *dzf, *dzf_inv, *dzc_inv, *dzflzi;
     long long sy, sz;  // strides of j and k
+    int kc;            // planes per k chunk of the stencil tiles (0: default)
 };
 
 __host__ inline G make_g(const pll_grid_dev *g)
@@ -28,6 +29,7 @@ __host__ inline G make_g(const pll_grid_dev *g)
     r.re_inv = g->re_inv;
     r.dt = g->dt;
     r.u_crf = g->u_crf;
+    r.kc = g->rk_kc;
     r.dzf = g->dzf;
     r.dzf_inv = g->dzf_inv;
     r.dzc_inv = g->dzc_inv;

@@ -173,6 +173,7 @@ typedef struct {
     int nx, ny, nz;
     double dx_inv, dy_inv, re_inv, dt, u_crf;
     const double *dzf, *dzf_inv, *dzc_inv, *dzflzi; /* [nz+2] */
+    int rk_kc;                 /* planes per k chunk of the stencil tiles (0: 64) */
 } pll_grid_dev;
 
 /* body: NULL, or three device fields (haloed layout) added to the momentum

@@ -757,7 +757,7 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -3; }
 Tiles rk_tiles(const G &g, int ty = RK_TY)
 {
     Tiles t;
-    t.kc = 64;
+    t.kc = g.kc > 0 ? g.kc : 64;
     t.ntx = (g.nx + RK_TX - 1) / RK_TX;
     t.nty = (g.ny + ty - 1) / ty;
     t.nkc = (g.nz + t.kc - 1) / t.kc;
