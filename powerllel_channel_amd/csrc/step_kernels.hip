@@ -109,9 +109,12 @@ __device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n
 // is written to LDS (the plane's stores may stay in flight), and RK2's old
 // values of plane k are loaded before the prefetch of plane k+2, so that
 // waiting for them leaves the prefetch in flight.
+// Occupancy hints: the FULL RK1 kernels at 4 waves per SIMD; the general RK2
+// kernels without a body force at 5 (96 VGPRs, no spills: the forced RK2 of
+// pll_channel_run had 100 and 4 waves; step 4.365 -> 4.350 ms at 768x384x256).
 template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY, bool FILL = false,
           bool FULL = false>
-__global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : 1)) k_rk(G g, Tiles t, const double *__restrict__ qu,
+__global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : (SECOND && !FULL && !BODY ? 5 : 1))) k_rk(G g, Tiles t, const double *__restrict__ qu,
                                                      const double *__restrict__ qv,
                                                      const double *__restrict__ qw,
                                                      double *__restrict__ ou,
