@@ -236,46 +236,49 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
 #pragma unroll
         for (int j = 0; j < H * M; ++j) pre[j] = src[j * jstride];
     };
+    const int s = tid >> lP, p = tid & (P - 1);
+    // eigenvalues of this thread's column in each group of a tile, loaded
+    // with the tile's values (so one wait covers both)
+    double lams[H];
+    auto load_lams = [&](int tile) {
+        const int my = tile / g.ntx, qt0 = (tile % g.ntx) * SW;
+        const int iy = my <= g.ny - my ? my : g.ny - my;
+#pragma unroll
+        for (int h = 0; h < H; ++h)
+            lams[h] = g.lamx[lam_ix(qt0 + h * S + s, my, g.L, g.ny)] + g.lamy[iy];
+    };
     int tile = (int)blockIdx.x < g.ntiles ? order[blockIdx.x] : 0;
-    if ((int)blockIdx.x < g.ntiles) fetch(tile);
-    // Results of the previous tile, held in registers until the current
-    // tile's values are in LDS: stores and loads share one in-order counter,
-    // so a store issued before that wait would be waited for with it.
-    St out[H * M];
-    St *__restrict__ out_base = nullptr;
-    // Every global load of an iteration is issued before the prefetch of the
-    // next tile, and the prefetch is unconditional (the last iteration
-    // re-reads its own tile), so that the wait for a load never includes the
-    // prefetch.
+    if ((int)blockIdx.x < g.ntiles) {
+        fetch(tile);
+        load_lams(tile);
+    }
+    // Order of the global accesses in an iteration: wait for this tile's
+    // values (and eigenvalues), issue the next tile's loads, solve, then store
+    // this tile's results straight from LDS.  The next iteration's wait for
+    // its values leaves those stores in flight (loads and stores share one
+    // in-order counter: the stores were issued after the loads waited for).
+    // The prefetch is unconditional (the last iteration re-reads its own
+    // tile), so that no wait depends on whether a next tile exists.
     for (int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
     const int tn = t + (int)gridDim.x;
     const int tile_next = order[tn < g.ntiles ? tn : t];
     const int my = tile / g.ntx, qt0 = (tile % g.ntx) * SW;
     St *__restrict__ base = Wr + (long long)my * g.nx + qt0;
     const int iy = my <= g.ny - my ? my : g.ny - my;
-    const int s = tid >> lP, p = tid & (P - 1);
-    double lams[H];
-#pragma unroll
-    for (int h = 0; h < H; ++h)
-        lams[h] = g.lamx[lam_ix(qt0 + h * S + s, my, g.L, g.ny)] + g.lamy[iy];
     __syncthreads();  // the previous tile's results have been read from LDS
 #pragma unroll
     for (int j = 0; j < H * M; ++j) tl[swz(kt + (j << lPH), st, S, SW, M)] = (double)pre[j];
     __syncthreads();
-    // lam is needed from here on: waiting for it now, before the stores and
-    // the prefetch are issued, keeps those out of that wait
+    double lamc[H];
 #pragma unroll
-    for (int h = 0; h < H; ++h) asm volatile("" : : "v"(lams[h]));
-    if (out_base) {
-#pragma unroll
-        for (int j = 0; j < H * M; ++j) out_base[j * jstride] = out[j];
-    }
+    for (int h = 0; h < H; ++h) lamc[h] = lams[h];
     fetch(tile_next);
+    load_lams(tile_next);
 
 #pragma unroll 1
     for (int h = 0; h < H; ++h) {
     const int q = qt0 + h * S + s;
-    const double lam = (H == 1 || h == 0) ? lams[0] : lams[H - 1];
+    const double lam = (H == 1 || h == 0) ? lamc[0] : lamc[H - 1];
     if (qt0 < g.qx && iy < g.iyx) {
         // The reference's recurrences with the stored pivots (low modes, where
         // the block elimination below loses accuracy):
@@ -472,14 +475,10 @@ __global__ void __launch_bounds__(512) k_trid_part(St *__restrict__ Wr,
     }
     }  // h
     __syncthreads();
+    St *__restrict__ dst = base + toff;
 #pragma unroll
-    for (int j = 0; j < H * M; ++j) out[j] = (St)tl[swz(kt + (j << lPH), st, S, SW, M)];
-    out_base = base + toff;
+    for (int j = 0; j < H * M; ++j) dst[j * jstride] = (St)tl[swz(kt + (j << lPH), st, S, SW, M)];
     tile = tile_next;
-    }
-    if (out_base) {
-#pragma unroll
-        for (int j = 0; j < H * M; ++j) out_base[j * jstride] = out[j];
     }
 }
 
@@ -514,7 +513,9 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
     if (d->tp.P > 0) {
         // FP32 storage with wide tiles: H = 2 groups of S columns per tile
         const int H = (sizeof(St) == sizeof(float) && d->tp.h32 == 2) ? 2 : 1;
-        constexpr int HW = sizeof(St) == sizeof(float) ? 2 : 1;  // H = 2 instantiated for FP32 only
+        // H = 2 is instantiated for FP32 storage only (FP64 values of a
+        // 64-column tile need more registers than a wave has: it spills)
+        constexpr int HW = sizeof(St) == sizeof(float) ? 2 : 1;
         PArgs g;
         g.a = d->a;
         g.c = d->c;
