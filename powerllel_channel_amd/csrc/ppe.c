@@ -249,6 +249,10 @@ static int choose_launch(pll_ppe *h)
     const pll_fftplan *px = &h->d.px, *py = &h->d.py;
     int maxt = pll_fft_max_threads(px->E);
     int rw = 256 / px->T;
+    {
+        const char *pr = getenv("PLL_X_ROWS"); /* tuning override of the rows per workgroup */
+        if (pr && atoi(pr) > 0) rw = atoi(pr);
+    }
     if (rw < 1) rw = 1;
     while (rw > 1 && (rw * px->T > maxt || (size_t)rw * (px->L + 1) * 16 > 65536)) rw--;
     if (rw * px->T > maxt || (size_t)rw * (px->L + 1) * 16 > 65536)
