@@ -241,6 +241,62 @@ def bench_dropin_host(steps=6, warmup=2):
             "sample": f"median of {steps} calls after {warmup}, host numpy array"}
 
 
+def bench_adjoint(torch, dev, reps=3):
+    """The differentiation form on its workload: 256x192x128, nu = 1e-4,
+    dt = 2e-4, a 20-step window with checkpoints every 5 steps, body force
+    N(0, 1e-3^2) per cell; mean wall shear J and its gradient with respect to
+    the body force.  Median ms of `reps` forward and backward passes after
+    one of each."""
+    from powerllel_channel_amd import Channel, PoissonSolver
+    from powerllel_channel_amd.adjoint import WallShearWindow
+    from powerllel_channel_amd.workloads import CASES, poiseuille_fields
+    case = CASES["adjoint_256x192x128"]
+    g = case.grid()
+    ch = Channel(g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.dzf, case.nu, case.dt,
+                 is_forced=(True, False, False), vel_force=(1.0, 0.0, 0.0))
+    s = PoissonSolver(g.nx, g.ny, g.nz, g.dx, g.dy, g.dzf)
+    u, v, w = poiseuille_fields(g, 3)
+    U = [torch.from_numpy(a).to(dev) for a in (u, v, w)]
+    ch.bc_vel(*U)
+    rng = np.random.default_rng(103)
+    B = []
+    for _ in range(3):
+        b = np.zeros(g.shape)
+        b[1:-1, 1:-1, 1:-1] = 1e-3 * rng.standard_normal((g.nz, g.ny, g.nx))
+        B.append(torch.from_numpy(b).to(dev))
+    G = [torch.empty(g.shape, dtype=torch.float64, device=dev) for _ in range(3)]
+    win = WallShearWindow(ch, s, case.steps, 5)
+    tf, tb = [], []
+    J = 0.0
+    for it in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = win.forward(*U, *B)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        win.backward(*G)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it:
+            tf.append((t1 - t0) * 1e3)
+            tb.append((t2 - t1) * 1e3)
+    gnorm = float(max(float(x.abs().max()) for x in G))
+    dev_bytes = win.device_bytes()
+    win.close()
+    ch.close()
+    s.close()
+    assert np.isfinite(J) and np.isfinite(gnorm), "non-finite adjoint"
+    msf, msb = float(np.median(tf)), float(np.median(tb))
+    log(f"[bench] adjoint {g.nx}x{g.ny}x{g.nz}, {case.steps} steps: forward {msf:.2f} ms, "
+        f"backward {msb:.2f} ms ({msb / msf:.2f}x), J = {J:.6e}")
+    return {"workload": f"adjoint_{g.nx}x{g.ny}x{g.nz}", "steps": case.steps, "ckpt_every": 5,
+            "forward_ms": round(msf, 3), "backward_ms": round(msb, 3),
+            "backward_over_forward": round(msb / msf, 3),
+            "ms_per_step_forward_and_backward": round((msf + msb) / case.steps, 4),
+            "J": J, "grad_max_abs": gnorm, "window_device_bytes": dev_bytes,
+            "sample": f"median of {reps} forward and backward passes after one of each"}
+
+
 def copy_bandwidth(torch, dev):
     """Device-to-device copy of 2 GiB (read + write), for context."""
     n = 2 * 1024 ** 3 // 8
@@ -335,6 +391,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ppe-only", action="store_true", help="skip the RK2 step timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window timing")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the drop-in call on a host array")
     args = ap.parse_args()
@@ -404,6 +461,8 @@ def main():
                                "bytes_per_cell": RUN_STEP_BYTES_PER_CELL,
                                "effective_GBs_312": round(eff_gbs, 1),
                                "effective_frac_312": round(eff_gbs / HBM_PEAK_GBS, 4)}
+    if not args.ppe_only and not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(torch, dev)
     if not args.no_dropin:
         out["dropin_host"] = bench_dropin_host()
     out["copy_bw_GBs"] = round(copy_bandwidth(torch, dev), 1)
