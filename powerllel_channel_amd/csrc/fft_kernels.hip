@@ -474,6 +474,7 @@ __device__ __forceinline__ void stages_ct(V (&v)[E], int t, const V *__restrict_
 template <int L_, int E_, int... Rs>
 struct CtFFT {
     static constexpr int E = E_;
+    static constexpr int kT = L_ / E_;  // threads per row
     static constexpr bool kSplit = true;
     static __device__ __forceinline__ int threads(const pll_fftplan &) { return L_ / E_; }
     static __device__ __forceinline__ int length(const pll_fftplan &) { return L_; }
@@ -489,6 +490,7 @@ struct CtFFT {
 template <int E_>
 struct RtFFT {
     static constexpr int E = E_;
+    static constexpr int kT = 0;  // threads per row known at run time only
     static constexpr bool kSplit = false;
     static __device__ __forceinline__ int threads(const pll_fftplan &P) { return P.T; }
     static __device__ __forceinline__ int length(const pll_fftplan &P) { return P.L; }
@@ -507,16 +509,40 @@ constexpr size_t lds_elem()
     return F::kSplit ? sizeof(typename Sc<V>::T) : sizeof(V);
 }
 
+// The x kernels' rows are T consecutive lanes of one wavefront when T is a
+// compile-time power of two up to 64: their mirror can then go through lane
+// shuffles instead of LDS (no barriers).
+template <class F>
+constexpr bool shfl_mirror()
+{
+    return F::kT > 0 && F::kT <= 64 && (F::kT & (F::kT - 1)) == 0;
+}
+
 // mv[e] := value at position n - m of the row (m = t + eT; position 0 for
 // m = 0), where v[e] holds the value at m: the mirror the real-to-complex
-// packing needs.  Every thread of the workgroup calls it (barriers); the
-// caller synchronises before the row segment is written again.
-template <class F, typename V, int E>
+// packing needs.  LDS form: every thread of the workgroup calls it
+// (barriers); the caller synchronises before the row segment is written
+// again.  SHFL form (rows of T consecutive lanes, shfl_mirror<F>()): for
+// t >= 1 position n - m = (T - t) + (E - 1 - e) T is in lane T - t of the
+// row, register E - 1 - e; for t = 0 it is (E - e) T, this lane's register
+// (E - e) mod E.  No LDS, no barrier.
+template <class F, typename V, int E, bool SHFL = false>
 __device__ __forceinline__ void mirror(const V (&v)[E], int t, int T, int n, unsigned char *row,
                                        V (&mv)[E])
 {
     using S = typename Sc<V>::T;
-    if constexpr (F::kSplit) {
+    if constexpr (SHFL) {
+        static_assert(shfl_mirror<F>(), "rows of T consecutive lanes, T a power of two <= 64");
+        constexpr int TT = F::kT;
+        const int src = (int)(threadIdx.x & 63) - t + ((TT - t) & (TT - 1));
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const V a = v[E - 1 - e];
+            const S ax = __shfl(a.x, src), ay = __shfl(a.y, src);
+            const V b = v[(E - e) % E];
+            mv[e] = t == 0 ? b : mk<V>(ax, ay);
+        }
+    } else if constexpr (F::kSplit) {
         S *ls = reinterpret_cast<S *>(row);
 #pragma unroll
         for (int e = 0; e < E; ++e) ls[t + e * T] = v[e].x;
@@ -1012,9 +1038,13 @@ k_xfwd(Src src, V *__restrict__ W, const V *__restrict__ twr, XArgs<V> a)
         }
     }
     F::template run<-1, 2>(v, t, a.P, a.tw, ml);
-    __syncthreads();
     V zm[E];
-    mirror<F, V, E>(v, t, T, L, ml, zm);
+    if constexpr (shfl_mirror<F>()) {
+        mirror<F, V, E, true>(v, t, T, L, ml, zm);
+    } else {
+        __syncthreads();
+        mirror<F, V, E>(v, t, T, L, ml, zm);
+    }
     V *dst = W + row * L;
     const V wt = twr[t];
 #pragma unroll
@@ -1057,7 +1087,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? src[t + e * T] : mk<V>(S(0), S(0));
     V xm[E];
-    mirror<F, V, E>(v, t, T, L, ml, xm);
+    mirror<F, V, E, shfl_mirror<F>()>(v, t, T, L, ml, xm);
     const V wt = twr[t];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -1073,7 +1103,7 @@ k_xbwd(const V *__restrict__ W, Dst dst, const V *__restrict__ twr, XArgs<V> a)
             v[e] = mk<V>(Sm.x - Tt.y, Sm.y + Tt.x);  // S + i Tt
         }
     }
-    __syncthreads();
+    if constexpr (!shfl_mirror<F>()) __syncthreads();  // the mirror's LDS reads are done
     F::template run<+1, 2>(v, t, a.P, a.tw, ml);
     if constexpr (!NORM && Dst::kAligned && sizeof(S) == sizeof(double)) {
         // x_{2m+1} of each pair through LDS to the next lane: stores of
