@@ -265,11 +265,15 @@ __global__ void __launch_bounds__(256) k_low_y(const double2 *__restrict__ rx,
 // The twelve real systems (mode q = 0..5, real and imaginary part) in one
 // workgroup: the multipliers m_k = a_k / piv_{k-1} and 1 / piv_k of the four
 // pivot columns (kx, |ky|) in {0,1}^2 are formed in parallel into LDS with
-// the right-hand sides; then one lane per system runs the recurrences of
-// ssolve_trid (src/src_c/mod_poisson_solver.c:319-356) with the stored
-// pivots, forward y_k = d_k - m_k y_{k-1}, top x = y / piv (0 for a zero
-// pivot and for the pinned (0,0) mode), backward x_k = (y_k - c_k x_{k+1}) / piv_k,
-// each step one multiply-add on LDS values.
+// the right-hand sides; then the recurrences of ssolve_trid
+// (src/src_c/mod_poisson_solver.c:319-356) with the stored pivots, forward
+// y_k = d_k - m_k y_{k-1}, top x = y / piv (0 for a zero pivot and for the
+// pinned (0,0) mode), backward x_k = (y_k - c_k x_{k+1}) / piv_k.  Each
+// system has 16 lanes of one wavefront (four systems per wavefront), lane p
+// owning rows [p M, (p+1) M): both recurrences are affine in the value that
+// enters a lane's rows, so a shuffle scan of the lanes' affine maps gives
+// every lane its entering value, and the lane then runs the recurrence over
+// its rows as written.
 __global__ void __launch_bounds__(256) k_low_z(const double2 *__restrict__ ly,
                                                double2 *__restrict__ lx,
                                                const double *__restrict__ piv,
@@ -295,22 +299,70 @@ __global__ void __launch_bounds__(256) k_low_z(const double2 *__restrict__ ly,
     }
     for (int k = t; k < nz; k += blockDim.x) cc[k] = c[k];
     __syncthreads();
-    if (t >= 12) return;
-    const int q = t >> 1, kx = q / 3, kyq = q % 3;
+    constexpr int P = 16;
+    const int sys = t / P, p = t % P;  // sys < 12 for t < 192; lanes 192.. idle
+    const int q = sys >> 1, kx = q / 3, kyq = q % 3;
     const int cls = kx + 2 * (kyq == 0 ? 0 : 1);
+    const bool act = sys < 12;
     const double *m = mk + cls * nz, *r = ip + cls * nz;
-    double prev = rhs[t];
-    for (int k = 1; k < nz; ++k) {
-        prev = rhs[12 * k + t] - m[k] * prev;
-        rhs[12 * k + t] = prev;
+    const int M = (nz + P - 1) / P;
+    const int k0 = min(p * M, nz), k1 = min(k0 + M, nz);  // rows [k0, k1), maybe empty
+    const int lane = t & 63;
+    // forward: lane map y(k1 - 1) = al + be y(k0 - 1)
+    double al = 0.0, be = 1.0;
+    if (act) {
+        for (int k = k0; k < k1; ++k) {
+            al = rhs[12 * k + sys] - m[k] * al;
+            be = -m[k] * be;
+        }
     }
+    for (int h = 1; h < P; h *= 2) {
+        const double al2 = __shfl(al, p >= h ? lane - h : lane);
+        const double be2 = __shfl(be, p >= h ? lane - h : lane);
+        if (p >= h) {
+            al = al + be * al2;
+            be = be * be2;
+        }
+    }
+    double y = __shfl(al, p > 0 ? lane - 1 : lane);  // y(k0 - 1); m_0 = 0 makes y(-1) irrelevant
+    if (p == 0) y = 0.0;
+    if (act) {
+        for (int k = k0; k < k1; ++k) {
+            y = rhs[12 * k + sys] - m[k] * y;
+            rhs[12 * k + sys] = y;
+        }
+    }
+    // backward: lane map x(k0) = ga + de x(k1); the top row from the top rule
     const bool pin = pin_gauge && q == 0;
-    double next = pin ? 0.0 : prev * r[nz - 1];  // r = 0 for a zero pivot
-    double *dst = reinterpret_cast<double *>(lx) + t;
-    dst[12 * (long long)(nz - 1)] = next;
-    for (int k = nz - 2; k >= 0; --k) {
-        next = (rhs[12 * k + t] - cc[k] * next) * r[k];
-        dst[12 * (long long)k] = next;
+    double ga = 0.0, de = 1.0;
+    if (act) {
+        for (int k = k1 - 1; k >= k0; --k) {
+            if (k == nz - 1) {
+                ga = pin ? 0.0 : rhs[12 * k + sys] * r[k];  // r = 0 for a zero pivot
+                de = 0.0;
+            } else {
+                ga = (rhs[12 * k + sys] - cc[k] * ga) * r[k];
+                de = -(cc[k] * de) * r[k];
+            }
+        }
+    }
+    for (int h = 1; h < P; h *= 2) {
+        const double ga2 = __shfl(ga, p + h < P ? lane + h : lane);
+        const double de2 = __shfl(de, p + h < P ? lane + h : lane);
+        if (p + h < P) {
+            ga = ga + de * ga2;
+            de = de * de2;
+        }
+    }
+    double x = __shfl(ga, p < P - 1 ? lane + 1 : lane);  // x(k1); unused by the top row
+    if (p == P - 1) x = 0.0;
+    if (act) {
+        double *dst = reinterpret_cast<double *>(lx) + sys;
+        for (int k = k1 - 1; k >= k0; --k) {
+            x = k == nz - 1 ? (pin ? 0.0 : rhs[12 * k + sys] * r[k])
+                            : (rhs[12 * k + sys] - cc[k] * x) * r[k];
+            dst[12 * (long long)k] = x;
+        }
     }
 }
 
