@@ -9,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("nx,ny,nz,sweeps", [(128, 96, 48, -1), (128, 96, 48, 1),
-                                             (256, 128, 64, 2)])
+                                             (256, 128, 64, 2), (768, 384, 256, -1),
+                                             (768, 384, 256, 1)])
 def test_solve_replays_in_graph(cuda, nx, ny, nz, sweeps):
     import torch
     from powerllel_channel_amd import PoissonSolver
