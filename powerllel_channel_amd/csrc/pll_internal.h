@@ -61,6 +61,7 @@ typedef struct {
         const double *lamy;    /* [ny/2+1] */
         const int *order;      /* [tiles] launch order, exact tiles first */
         int h32;               /* FP32 storage: 2 = tiles of 2 S columns (0/1: S) */
+        int h64;               /* FP64 storage: the same, at S = 16 only */
         int qx32;              /* qx for those tiles */
         const int *order32;    /* [nx/(2S) ny] their launch order */
     } tp;

@@ -536,6 +536,16 @@ int pll_ppe_create(pll_ppe **out, int nx, int ny, int nz, double dx, double dy,
                 }
             }
             h->d.tp.h32 = h32;
+            {
+                /* FP64 storage: two 16-column groups make the 256-byte row
+                 * segments that 32-column tiles give where P = 32 lanes per
+                 * column leave room for 16 columns only (nz = 384); at M = 16
+                 * rows per lane the second group's values would spill
+                 * (PLL_TRID64_H=1: one group) */
+                const char *p64 = getenv("PLL_TRID64_H");
+                h->d.tp.h64 =
+                    (h32 == 2 && S == 16 && M <= 12 && !(p64 && atoi(p64) == 1)) ? 2 : 1;
+            }
             h->d.tp.qx32 = qx32;
             h->d.tp.order32 = (const int *)h->order32;
             double *lxh = (double *)malloc(sizeof(double) * (size_t)(L + 1));

@@ -512,10 +512,10 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
 {
     if (d->tp.P > 0) {
         // FP32 storage with wide tiles: H = 2 groups of S columns per tile
-        const int H = (sizeof(St) == sizeof(float) && d->tp.h32 == 2) ? 2 : 1;
-        // H = 2 is instantiated for FP32 storage only (FP64 values of a
-        // 64-column tile need more registers than a wave has: it spills)
-        constexpr int HW = sizeof(St) == sizeof(float) ? 2 : 1;
+        const int H = ((sizeof(St) == sizeof(float) ? d->tp.h32 : d->tp.h64) == 2) ? 2 : 1;
+        // H = 2: FP32 storage at S = 16 and 32, FP64 storage at S = 16 only (FP64
+        // values of a 64-column tile need more registers than a wave has)
+        constexpr int HW32 = sizeof(St) == sizeof(float) ? 2 : 1;
         PArgs g;
         g.a = d->a;
         g.c = d->c;
@@ -561,13 +561,13 @@ int launch_trid(const pll_ppe_dev *d, void *W, hipStream_t s)
         case 8: PLL_TRID_LAUNCH(M_, 8, D_, 1); break;                                           \
         case 16:                                                                                \
             if (H == 2)                                                                         \
-                PLL_TRID_LAUNCH(M_, 16, D_, HW);                                                \
+                PLL_TRID_LAUNCH(M_, 16, D_, 2);                                                \
             else                                                                                \
                 PLL_TRID_LAUNCH(M_, 16, D_, 1);                                                 \
             break;                                                                              \
         case 32:                                                                                \
             if (H == 2)                                                                         \
-                PLL_TRID_LAUNCH(M_, 32, D_, HW);                                                \
+                PLL_TRID_LAUNCH(M_, 32, D_, HW32);                                                \
             else                                                                                \
                 PLL_TRID_LAUNCH(M_, 32, D_, 1);                                                 \
             break;                                                                              \
