@@ -375,6 +375,33 @@ def test_fp32_wide_z_tiles(cuda, nx, ny, nz, sr, monkeypatch):
     assert e0 <= 1e-6 and e2 <= 1e-11
 
 
+@pytest.mark.parametrize("nx,ny,nz,sr", [(96, 64, 384, 4.4), (64, 48, 384, 4.0)])
+def test_fp64_two_group_z_tiles(cuda, nx, ny, nz, sr, monkeypatch):
+    """At nz = 384 the FP64 z solve takes tiles of two 16-column groups;
+    against one group (PLL_TRID64_H=1) the answers agree to the solve's
+    accuracy (a wider first tile sends a few more low-mode columns through
+    the exact recurrence)."""
+    import torch
+    from powerllel_channel_amd import PoissonSolver
+    from powerllel_channel_amd.mesh import Grid
+    from powerllel_channel_amd.workloads import LX, LY, LZ, spectral_rhs
+    print(f"[fp64 two-group z tiles] {nx}x{ny}x{nz}", flush=True)
+    g = Grid(nx, ny, nz, LX, LY, LZ, sr)
+    rhs = spectral_rhs(g, 6)
+    out = {}
+    for h in ("1", "2"):
+        monkeypatch.setenv("PLL_TRID64_H", h)
+        s = PoissonSolver(nx, ny, nz, g.dx, g.dy, g.dzf)
+        p = torch.from_numpy(rhs).to(cuda)
+        s.solve(p)
+        torch.cuda.synchronize()
+        out[h] = p.cpu().numpy()
+        s.close()
+    e = _rel(out["2"], out["1"])
+    print(f"  two groups vs one: {e:.2e}")
+    assert e <= TOL
+
+
 def test_refined_low_modes_one_sweep(cuda, monkeypatch):
     """768x384x256, stretch 4: with the six lowest (kx, ky) modes of each FP32
     solve projected, solved and added back in FP64, one sweep reaches the
