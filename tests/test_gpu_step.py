@@ -317,6 +317,28 @@ def test_100_steps_192(orc, cuda):
     assert div < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(48, 32, 24), (40, 36, 21), (32, 96, 96), (40, 90, 100)])
+def test_maxloc_row_orders(orc, cuda, shape):
+    """calcMaxDiv and calcMaxCFL against the oracle, value and location, with the
+    rows handed out as y slabs per XCD (ny a multiple of 8) and in plain order
+    (ny = 36, 90), and with more than 8192 row partials (the final reduction's
+    eight-at-a-time loop); a zero field makes every cell tie, so the location
+    must be the lowest cell index whatever order the rows were reduced in."""
+    import torch
+    nx, ny, nz = shape
+    print(f"[maxloc] {shape}", flush=True)
+    g, G, ch, f = _setup(orc, nx, ny, nz, 3.0, seed=11)
+    u, v, w = f[0], f[1], f[2]
+    du, dv, dw = _dev(torch, cuda, u, v, w)
+    assert ch.max_div(du, dv, dw) == orc.max_div(G, u, v, w)
+    assert ch.max_cfl(du, dv, dw) == orc.max_cfl(G, u, v, w)
+    z = np.zeros(g.shape)
+    dz = _dev(torch, cuda, z, z, z)
+    val, ijk = ch.max_div(*dz)
+    assert (val, ijk) == orc.max_div(G, z, z, z)
+    assert val == 0.0
+
+
 def test_monitors_vs_oracle(orc, cuda):
     """calcMaxCFL / calcMaxDiv / checkNaN on the GPU against the oracle, and the
     run-time check of main.f90:213-222 (NaN planted in the interior and in a
