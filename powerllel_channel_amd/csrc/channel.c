@@ -33,7 +33,6 @@ struct pll_channel {
     long long *sidx;   /* device: 1 index */
     double *dzf_h;     /* host copy of dzf_global (nz+2) */
     double lx, ly, lz;
-    double *run;       /* device: 3 haloed fields, the second velocity of pll_channel_run */
 };
 
 static int kl(int rc, const char *what)
@@ -51,7 +50,6 @@ int pll_channel_destroy(pll_channel *c)
     if (c->pidx) hipFree(c->pidx);
     if (c->scal) hipFree(c->scal);
     if (c->sidx) hipFree(c->sidx);
-    if (c->run) hipFree(c->run);
     free(c->dzf_h);
     free(c);
     return PLL_OK;
@@ -357,66 +355,68 @@ int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v, double 
     return step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream);
 }
 
-/* A step of a chain up to its pressure solve: RK1 of the velocity `from`
- * (from == NULL: the caller's projected X, plain RK1; otherwise an
- * unprojected RK2 output, projected as RK1 reads it, pll_launch_rk1_corr,
- * with the projected interior written to `to`), then RK2 in place in `to`,
- * the forcing means and the FP64 solve with calcRHS formed in its x
- * transform.  The projection of `to` and its boundary fill are left to the
+/* A step of a chain up to its pressure solve, in place on X = (u, v, w):
+ * proj == 0: X is projected (the caller's state), plain RK1 and RK2;
+ * proj == 1: X is the previous step's RK2 output, unprojected (its boundary
+ * fill not made either), and p that step's pressure: RK1 projects every
+ * value it reads (pll_launch_rk1_corr, writing only the RK1 stage) and RK2
+ * projects its old value at each cell as it reads it (pll_launch_rk2_proj),
+ * both from p and the forcing means of the previous step.  Then the forcing
+ * means of the new RK2 output and the FP64 solve with calcRHS formed in its
+ * x transform.  The projection of X and its boundary fill are left to the
  * next step (or to the chain's final projection). */
-static int chain_step(pll_channel *c, pll_ppe *ppe, double *const *from, double *const *to,
-                      double *u1, double *v1, double *w1, double *p, int fmask, hipStream_t s)
+static int chain_step(pll_channel *c, pll_ppe *ppe, int proj, double *const *X, double *u1,
+                      double *v1, double *w1, double *p, int fmask, hipStream_t s)
 {
     const int nb = pll_rk_blocks(&c->g);
     const double *mean = fmask ? c->scal + 4 : NULL;
     const double dti = 1.0 / c->g.dt;
     int rc;
-    if (!from) {
-        if ((rc = kl(pll_launch_rk1(&c->g, to[0], to[1], to[2], u1, v1, w1, NULL, s), "rk1")))
+    if (!proj) {
+        if ((rc = kl(pll_launch_rk1(&c->g, X[0], X[1], X[2], u1, v1, w1, NULL, s), "rk1")))
             return rc;
-    } else if ((rc = kl(pll_launch_rk1_corr(&c->g, from[0], from[1], from[2], p, to[0], to[1],
-                                            to[2], u1, v1, w1, fmask, c->vel_force[0],
-                                            c->vel_force[1], mean, s),
-                        "rk1 (projecting)"))) {
-        return rc;
+        if ((rc = kl(pll_launch_rk2(&c->g, X[0], X[1], X[2], u1, v1, w1, fmask ? c->part : NULL,
+                                    fmask, NULL, 1, s),
+                     "rk2")))
+            return rc;
+    } else {
+        if ((rc = kl(pll_launch_rk1_corr(&c->g, X[0], X[1], X[2], p, NULL, NULL, NULL, u1, v1,
+                                         w1, fmask, c->vel_force[0], c->vel_force[1], mean, s),
+                     "rk1 (projecting)")))
+            return rc;
+        if ((rc = kl(pll_launch_rk2_proj(&c->g, X[0], X[1], X[2], u1, v1, w1, p,
+                                         fmask ? c->part : NULL, fmask, c->vel_force[0],
+                                         c->vel_force[1], mean, s),
+                     "rk2 (projecting)")))
+            return rc;
     }
-    if ((rc = kl(pll_launch_rk2(&c->g, to[0], to[1], to[2], u1, v1, w1, fmask ? c->part : NULL,
-                                fmask, NULL, 1, s),
-                 "rk2")))
-        return rc;
     for (int d = 0; d < 2; d++)
         if ((fmask >> d) & 1)
             if ((rc = kl(pll_launch_mean_final(nb, c->part + (size_t)d * nb, c->scal + 4 + d,
                                                c->g.nx, c->g.ny, s),
                          "mean final")))
                 return rc;
-    /* no boundary fill of `to`: the solve's calcRHS and the next step's RK1
+    /* no boundary fill of X: the solve's calcRHS and the next step's RK1
      * read the values the fill would leave from the interior */
-    pll_div_src q = {to[0], to[1], to[2], c->g.dzf_inv, dti, dti * c->g.dx_inv,
+    pll_div_src q = {X[0], X[1], X[2], c->g.dzf_inv, dti, dti * c->g.dx_inv,
                      dti * c->g.dy_inv, 1};
     return pll_ppe_solve_div(ppe, &q, p, s);
 }
 
-/* m steps (m odd) whose projections are carried into the next step's RK1:
- * step n leaves its RK2 output (boundary-filled) and pressure p unprojected;
- * RK1 of step n+1 reads both and projects every value it reads, writing the
- * projected interior to the other velocity set, where RK2 then runs in place.
- * The sets alternate between the caller's (u, v, w) = X and c->run = Y; the
- * first step runs on X, so with m odd the last one ends on X and the final
- * projection (correctVel + forcing + boundary fill) is made in place there.
- * Same kernels and roundings as m calls of step_impl. */
+/* m steps whose projections are carried into the next step's RK1 and RK2:
+ * step n leaves its RK2 output and pressure p unprojected in place; the
+ * next step's RK1 and RK2 read both and project every value they read.
+ * After the last step the final projection (correctVel + forcing + boundary
+ * fill) is made in place.  Same kernels and roundings as m calls of
+ * step_impl. */
 static int run_chain(pll_channel *c, pll_ppe *ppe, int m, double *u, double *v, double *w,
                      double *u1, double *v1, double *w1, double *p, int fmask, hipStream_t s)
 {
-    const size_t n = (size_t)(c->g.nx + 2) * (c->g.ny + 2) * (c->g.nz + 2);
-    double *X[3] = {u, v, w}, *Y[3] = {c->run, c->run + n, c->run + 2 * n};
+    double *X[3] = {u, v, w};
     const double *mean = fmask ? c->scal + 4 : NULL;
     int rc;
-    if ((rc = chain_step(c, ppe, NULL, X, u1, v1, w1, p, fmask, s))) return rc;
-    for (int i = 0; i < (m - 1) / 2; i++) {
-        if ((rc = chain_step(c, ppe, X, Y, u1, v1, w1, p, fmask, s))) return rc;
-        if ((rc = chain_step(c, ppe, Y, X, u1, v1, w1, p, fmask, s))) return rc;
-    }
+    for (int i = 0; i < m; i++)
+        if ((rc = chain_step(c, ppe, i > 0, X, u1, v1, w1, p, fmask, s))) return rc;
     if ((rc = kl(pll_launch_correct_vel(&c->g, p, u, v, w, fmask, c->vel_force[0],
                                         c->vel_force[1], mean, s),
                  "correct_vel")))
@@ -446,19 +446,6 @@ int pll_channel_run(pll_channel *c, pll_ppe *ppe, int nsteps, double *u, double 
         for (int i = 0; i < nsteps; i++)
             if ((rc = step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream))) return rc;
         return PLL_OK;
-    }
-    if (!c->run) {
-        const size_t n = (size_t)(c->g.nx + 2) * (c->g.ny + 2) * (c->g.nz + 2);
-        hipError_t e = hipMalloc((void **)&c->run, sizeof(double) * 3 * n);
-        if (e != hipSuccess) {
-            c->run = NULL;
-            return pll_set_error(PLL_ENOMEM, "hipMalloc of the run's second velocity failed: %s",
-                                 hipGetErrorString(e));
-        }
-    }
-    if (nsteps % 2 == 0) {  /* one plain step, then an odd chain */
-        if ((rc = step_impl(c, ppe, u, v, w, u1, v1, w1, p, NULL, stream))) return rc;
-        nsteps--;
     }
     return run_chain(c, ppe, nsteps, u, v, w, u1, v1, w1, p, fmask, s);
 }

@@ -199,6 +199,13 @@ int pll_launch_rk1_corr(const pll_grid_dev *g, const double *u, const double *v,
 int pll_launch_rk2(const pll_grid_dev *g, double *u, double *v, double *w,
                    const double *u1, const double *v1, const double *w1, double *mpart,
                    int fmask, const double *const *body, int fill, hipStream_t s);
+/* RK2 of a multi-step run in place on (u, v, w), the previous step's
+ * unprojected RK2 output, projected with p (and the forcing shift from mean)
+ * as it is read; Mean_h partials into mpart as pll_launch_rk2 */
+int pll_launch_rk2_proj(const pll_grid_dev *g, double *u, double *v, double *w,
+                        const double *u1, const double *v1, const double *w1,
+                        const double *p, double *mpart, int fmask, double vfu,
+                        double vfv, const double *mean, hipStream_t s);
 int pll_rk_blocks(const pll_grid_dev *g);
 int pll_launch_calc_rhs(const pll_grid_dev *g, const double *u, const double *v,
                         const double *w, double *p, hipStream_t s);

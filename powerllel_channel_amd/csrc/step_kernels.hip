@@ -111,10 +111,16 @@ __device__ __forceinline__ int wrap_p(int i, int n) { return i == 0 ? n : (i > n
 // waiting for them leaves the prefetch in flight.
 // Occupancy hints: the FULL RK1 kernels at 4 waves per SIMD; the general RK2
 // kernels without a body force at 5 (96 VGPRs, no spills: the forced RK2 of
-// pll_channel_run had 100 and 4 waves; step 4.365 -> 4.350 ms at 768x384x256).
+// pll_channel_run had 100 and 4 waves; step 4.365 -> 4.350 ms at 768x384x256);
+// the PROJO RK2 at 4 (at 5 it spills).
+// PROJO (RK2 of a multi-step run): o is the previous step's RK2 output, not
+// yet projected; its projected value at the cell (correctVel with the forcing
+// shift, and w = 0 on the top interior row as updateBoundVel leaves it) is
+// formed as it is read, from o and p, in the roundings of the projecting RK1,
+// so that RK1 need not write the projected field.
 template <bool SECOND, bool MEAN, bool BODY, bool CORR = false, int TY = RK_TY, bool FILL = false,
-          bool FULL = false>
-__global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : (SECOND && !FULL && !BODY ? 5 : 1))) k_rk(G g, Tiles t, const double *__restrict__ qu,
+          bool FULL = false, bool PROJO = false>
+__global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) || PROJO ? 4 : (SECOND && !FULL && !BODY ? 5 : 1))) k_rk(G g, Tiles t, const double *__restrict__ qu,
                                                      const double *__restrict__ qv,
                                                      const double *__restrict__ qw,
                                                      double *__restrict__ ou,
@@ -152,8 +158,8 @@ __global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : (SECOND &&
         gjj[r] = gk[r] ? wrap(gj, g.ny) : 1;
     }
     // CORR: projected value of field f at interior (i, j, k), as k_correct
-    const double shu = (CORR && (cr.fmask & 1)) ? cr.vfu - cr.mean[0] : 0.0;
-    const double shv = (CORR && (cr.fmask & 2)) ? cr.vfv - cr.mean[1] : 0.0;
+    const double shu = ((CORR || PROJO) && (cr.fmask & 1)) ? cr.vfu - cr.mean[0] : 0.0;
+    const double shv = ((CORR || PROJO) && (cr.fmask & 2)) ? cr.vfv - cr.mean[1] : 0.0;
     auto proj = [&](int f, int ii, int jj, int kk) -> double {
         const long long e0 = ii + sy * jj + sz * kk;
         const double pc = cr.p[e0];
@@ -366,8 +372,20 @@ __global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : (SECOND &&
             fw = fw + fb.w[ek];
         }
         if (SECOND) {
-            const double ou0 = old ? old[0] : ou[ek], ov0 = old ? old[1] : ov[ek];
-            const double ow0 = old ? old[2] : ow[ek];
+            double ou0, ov0, ow0;
+            if constexpr (PROJO) {
+                // the expressions of proj() on o
+                const double pc = cr.p[ek];
+                ou0 = ou[ek] - (cr.p[ek + 1] - pc) * cr.dtdxi;
+                if (cr.fmask & 1) ou0 = ou0 + shu;
+                ov0 = ov[ek] - (cr.p[ek + sy] - pc) * cr.dtdyi;
+                if (cr.fmask & 2) ov0 = ov0 + shv;
+                ow0 = k == g.nz ? 0.0 : ow[ek] - (cr.p[ek + sz] - pc) * g.dt * g.dzc_inv[k];
+            } else {
+                ou0 = old ? old[0] : ou[ek];
+                ov0 = old ? old[1] : ov[ek];
+                ow0 = old ? old[2] : ow[ek];
+            }
             const double un = (n.u000 + g.dt * fu + ou0) * 0.5;
             const double vn = (n.v000 + g.dt * fv + ov0) * 0.5;
             ou[ek] = un;
@@ -384,7 +402,7 @@ __global__ void __launch_bounds__(RK_TX *TY, ((FULL && !SECOND) ? 4 : (SECOND &&
             ou[ek] = n.u000 + g.dt * fu;
             ov[ek] = n.v000 + g.dt * fv;
             ow[ek] = n.w000 + g.dt * fw;
-            if (CORR) {
+            if (CORR && cr.nu) {
                 cr.nu[ek] = n.u000;
                 cr.nv[ek] = n.v000;
                 cr.nw[ek] = n.w000;
@@ -905,6 +923,37 @@ extern "C" int pll_launch_rk2(const pll_grid_dev *gd, double *u, double *v, doub
                                mean ? fmask : 0, fb, Corr{});
         return ok();
     });
+}
+
+// RK2 of a multi-step run, in place: o = (u, v, w) is the previous step's
+// unprojected RK2 output, projected at each cell as it is read (PROJO), and
+// is overwritten with the new velocity; q = the RK1 stage; the boundary values
+// of the RK1 stage are read from its interior (FILL).
+extern "C" int pll_launch_rk2_proj(const pll_grid_dev *gd, double *u, double *v, double *w,
+                                   const double *u1, const double *v1, const double *w1,
+                                   const double *p, double *mpart, int fmask, double vfu,
+                                   double vfv, const double *mean, hipStream_t s)
+{
+    const G g = make_g(gd);
+    const Tiles t = rk_tiles(g);
+    const dim3 grid((unsigned)(t.ntx * t.nty * t.nkc)), blk(RK_TX * RK_TY);
+    Corr cr;
+    cr.p = p;
+    cr.nu = cr.nv = cr.nw = nullptr;
+    cr.dtdxi = g.dt * g.dx_inv;  // as pll_launch_correct_vel
+    cr.dtdyi = g.dt * g.dy_inv;
+    cr.fmask = mean ? fmask : 0;
+    cr.vfu = vfu;
+    cr.vfv = vfv;
+    cr.mean = mean;
+    const Body fb{nullptr, nullptr, nullptr};
+    if (mpart && fmask)
+        hipLaunchKernelGGL((k_rk<true, true, false, false, RK_TY, true, false, true>), grid, blk, 0,
+                           s, g, t, u1, v1, w1, u, v, w, mpart, fmask, fb, cr);
+    else
+        hipLaunchKernelGGL((k_rk<true, false, false, false, RK_TY, true, false, true>), grid, blk,
+                           0, s, g, t, u1, v1, w1, u, v, w, nullptr, 0, fb, cr);
+    return ok();
 }
 
 extern "C" int pll_launch_calc_rhs(const pll_grid_dev *gd, const double *u, const double *v,
