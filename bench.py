@@ -36,10 +36,11 @@ PPE_BYTES_PER_CELL = 80
 # RK1 48 + RK2 72 + calcRHS 32 + PPE 80 + correctVel 56 + forceVel 24: the
 # reference's unfused step
 STEP_BYTES_PER_CELL = 312
-# what one step of pll_channel_run moves: projecting RK1 80 (reads u, v, w, p,
-# writes the projected velocity and the RK1 stage) + RK2 72 + calcRHS fused
-# into the x transform 32 + the other four PPE stages 64
-RUN_STEP_BYTES_PER_CELL = 248
+# what one step of pll_channel_run moves: projecting RK1 56 (reads u, v, w, p,
+# writes the RK1 stage) + projecting RK2 80 (reads the RK1 stage, u, v, w and
+# p, writes u, v, w) + calcRHS fused into the x transform 32 + the other four
+# PPE stages 64
+RUN_STEP_BYTES_PER_CELL = 232
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_ppe_768x384x256.txt")
 
 
