@@ -203,9 +203,9 @@ int pll_channel_step(pll_channel *c, pll_ppe *ppe, double *u, double *v,
  * per-step checks and output), with the same results bit for bit.  With the
  * FP64 solve, w not forced and a large grid (at least 2048 RK tiles of
  * 64 x 4 x 64 cells), each step's projection (correctVel, forcing, boundary
- * fill) is formed inside the next step's RK1 as it reads the velocity, which
- * saves one pass over the fields per step; that needs three more haloed
- * fields of device memory (allocated on first use). */
+ * fill) is formed inside the next step's RK1 and RK2 as they read the
+ * velocity, which saves one pass over the fields per step; the run works in
+ * place and allocates no device memory. */
 int pll_channel_run(pll_channel *c, pll_ppe *ppe, int nsteps, double *u,
                     double *v, double *w, double *u1, double *v1, double *w1,
                     double *p, void *stream);
